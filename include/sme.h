@@ -1,0 +1,141 @@
+/*
+ * sme.h -- C ABI of the MI355X subgraph enumeration engine (libsme.so).
+ *
+ * Plain pointers and sizes only.  Every entry point returns an sme_status
+ * (0 = OK) and never exits the process; the message of the last failure of a
+ * context is available from sme_last_error().  A context is used from one
+ * host thread at a time (one context per device, like the reference's one
+ * host thread per device, src/gpu/executor/HybridSeparatePatternMatch.cuh:78).
+ *
+ * Reference interfaces each entry point replaces (paths in the reference
+ * tree, src/...):
+ *   sme_open / sme_close        CudaContextManager::CreateCudaContextManager +
+ *                               CnmemManagedCudaContext (gpu/utils/context/
+ *                               CudaContextManager.cuh:19-103,
+ *                               CudaContext.cuh:315-338) and the device
+ *                               memory limit kDeviceMemoryLimits (Meta.h:26-28)
+ *   sme_graph_upload            HybridGPUComponent::BuildDevicePartition ->
+ *                               DevGraphPartition::BuildIntraPartition
+ *                               (gpu/impl/common/HybridGPUComp.cuh:87-94,
+ *                               gpu/graph/DevGraphPartition.cuh:32-41)
+ *   sme_graph_destroy           HybridGPUComponent::ReleaseDevicePartition
+ *                               (HybridGPUComp.cuh:83-86)
+ *   sme_plan_create             InitGPUInstance plan selection
+ *                               (gpu/PatternMatchGPUInstance.cuh:120-146):
+ *                               Query + LazyTraversalPlan / TraversalPlan
+ *                               (query/common/Query.h:13-53,
+ *                               query/LazyTraversalPlan.h:13-29)
+ *   sme_plan_describe           LazyTraversalPlan::Print (LazyTraversalPlan.h:68-86)
+ *   sme_count                   HybridGPUComponent::GPUThreadExecute(IntraPartTask)
+ *                               (HybridGPUComp.cuh:53-69, task/Task.h:23-43)
+ *   sme_bin_read / sme_bin_write  GraphIO::ReadCSRBinFile / WriteCSRBinFile
+ *                               (graph/GraphIO.h:88-136, 356-387)
+ *   sme_snap_read               GraphIO::ReadSNAPFile (GraphIO.h:232-311)
+ *   sme_gen_rmat / sme_gen_er   no counterpart (the reference reads SNAP
+ *                               datasets); seeded synthetic inputs
+ *   sme_profile_*               GPUProfiler phase timers
+ *                               (gpu/utils/GPUProfiler.cuh:11-134)
+ */
+#ifndef SME_H_
+#define SME_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sme_ctx sme_ctx;
+typedef struct sme_graph sme_graph;
+typedef struct sme_plan sme_plan;
+
+enum sme_status {
+  SME_OK = 0,
+  SME_ERR_INVALID = 1,
+  SME_ERR_OOM = 2,
+  SME_ERR_HIP = 3,
+  SME_ERR_IO = 4,
+  SME_ERR_UNSUPPORTED = 5
+};
+
+/* GPU algorithms, numbered as Algo in the reference (Meta.h:47-66). */
+enum sme_algo { SME_ALGO_GPSM = 4, SME_ALGO_GPSM_REUSE = 5, SME_ALGO_LIGHT = 6 };
+
+/* LazyTraversalCompressLevel (query/common/LazyTraversalCommon.h:38-76). */
+enum sme_compress_level {
+  SME_CL_MATERIALIZE = 0,
+  SME_CL_NON_MATERIALIZE = 1,
+  SME_CL_NON_MATERIALIZE_OPT = 2,
+  SME_CL_SPECIAL = 3
+};
+
+#define SME_MAX_LEVELS 16
+
+typedef struct {
+  uint64_t batches;          /* level batches executed */
+  uint64_t d2h_syncs;        /* blocking device->host reads */
+  uint64_t peak_pool_bytes;  /* high-water mark of the intermediate pool */
+  double elapsed_ms;         /* host wall time of the call */
+} sme_stats;
+
+/* ---- context ---------------------------------------------------------- */
+/* Opens `device` and reserves `mem_budget_bytes` of device memory for
+ * intermediates (the reference's per-device limit, Meta.h:26-28). */
+int sme_open(int device, uint64_t mem_budget_bytes, sme_ctx** out);
+void sme_close(sme_ctx* ctx);
+const char* sme_last_error(const sme_ctx* ctx);
+int sme_device_count(int* out);
+
+/* ---- graph ------------------------------------------------------------ */
+/* Copies a CSR (BIN layout: row_ptrs[n+1] u64, cols[nnz] u32, every list
+ * strictly increasing) to the context's device.  The caller keeps its
+ * arrays. */
+int sme_graph_upload(sme_ctx* ctx, const uint64_t* row_ptrs, const uint32_t* cols, uint64_t n,
+                     uint64_t nnz, sme_graph** out);
+void sme_graph_destroy(sme_graph* g);
+
+/* ---- plan (host only) ------------------------------------------------- */
+int sme_plan_create(int algo, int query_type, int compress_level, int enable_ordering,
+                    sme_plan** out);
+/* Canonical text of the plan; returns the full length (may exceed cap). */
+int sme_plan_describe(const sme_plan* p, char* buf, size_t cap);
+/* Number of exec levels (length of the level_counts array of sme_count). */
+int sme_plan_levels(const sme_plan* p);
+void sme_plan_destroy(sme_plan* p);
+
+/* ---- execution -------------------------------------------------------- */
+/* Counts the embeddings whose first matched vertex lies in [v_begin, v_end).
+ * level_counts (nullable) receives, per exec level, the summed output size:
+ * partial instances after MATERIALIZE, candidates after COMPUTE /
+ * FILTER_COMPUTE / COMPUTE_PATH_COUNT, the count added by COUNT /
+ * COMPUTE_COUNT. */
+int sme_count(sme_ctx* ctx, const sme_graph* g, const sme_plan* p, uint32_t v_begin,
+              uint32_t v_end, uint64_t* out_count, uint64_t* level_counts, sme_stats* stats);
+
+/* ---- kernel profile (HIP events around every launch) ------------------ */
+int sme_profile_enable(sme_ctx* ctx, int on);
+int sme_profile_reset(sme_ctx* ctx);
+int sme_profile_slots(void);
+/* Per kernel family: name, launches, summed ms, algorithmic bytes, items. */
+int sme_profile_get(const sme_ctx* ctx, int slot, char* name, size_t cap, uint64_t* launches,
+                    double* ms, double* bytes, uint64_t* items);
+
+/* ---- host graph utilities (no device needed) --------------------------- */
+/* Arrays returned through row_ptrs / cols are released with sme_free_host. */
+int sme_bin_read(const char* path, uint64_t* n, uint64_t* nnz, uint64_t** row_ptrs,
+                 uint32_t** cols);
+int sme_bin_write(const char* path, const uint64_t* row_ptrs, const uint32_t* cols, uint64_t n,
+                  uint64_t nnz);
+int sme_snap_read(const char* path, int directed, uint64_t* n, uint64_t* nnz, uint64_t** row_ptrs,
+                  uint32_t** cols);
+int sme_gen_rmat(int scale, int edge_factor, double a, double b, double c, uint64_t seed,
+                 uint64_t* n, uint64_t* nnz, uint64_t** row_ptrs, uint32_t** cols);
+int sme_gen_er(uint64_t n_vertices, uint64_t n_edges, uint64_t seed, uint64_t* n, uint64_t* nnz,
+               uint64_t** row_ptrs, uint32_t** cols);
+void sme_free_host(void* p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SME_H_ */

@@ -1,0 +1,124 @@
+"""ctypes view of liboracle.so -- the CPU restatement of the reference path.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, and only as the checker.  See oracle.h for what
+each function restates (reference file:line) and oracle.c for how each one is
+pinned.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+REF_DIR = os.path.join(HERE, "_ref")
+REF_CPU = os.path.join(REF_DIR, "patternmatchcpu")
+REF_PLAN_DUMP = os.path.join(REF_DIR, "plan_dump")
+MAX_OPS = 16
+
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-C", HERE, "-s", "liboracle.so"], check=True)
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = ctypes.CDLL(LIB)
+        vp = ctypes.c_void_p
+        u64 = ctypes.c_uint64
+        L.or_light_plan.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]
+        L.or_plan_text.argtypes = [vp, ctypes.c_char_p, ctypes.c_size_t]
+        L.or_wcoj_count.argtypes = [u64, vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
+                                    ctypes.c_uint32, ctypes.c_int, ctypes.POINTER(u64)]
+        L.or_light_count.argtypes = [u64, vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
+                                     ctypes.POINTER(u64), ctypes.POINTER(u64)]
+        L.or_gpsm_count.argtypes = [u64, vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
+                                    ctypes.c_uint32, ctypes.c_int, ctypes.POINTER(u64),
+                                    ctypes.POINTER(u64)]
+        _lib = L
+    return _lib
+
+
+def plan_text(q: int, compress_level: int = 3, enable_ordering: bool = True) -> str:
+    L = lib()
+    p = ctypes.create_string_buffer(16384)
+    if L.or_light_plan(q, compress_level, int(enable_ordering), p) != 0:
+        raise ValueError("bad query / compress level")
+    buf = ctypes.create_string_buffer(16384)
+    L.or_plan_text(p, buf, 16384)
+    return buf.value.decode()
+
+
+def _arrays(rp, cols):
+    rp = np.ascontiguousarray(rp, dtype=np.uint64)
+    cols = np.ascontiguousarray(cols, dtype=np.uint32)
+    if cols.size == 0:
+        cols = np.zeros(1, dtype=np.uint32)
+    return rp, cols
+
+
+def wcoj_count(rp, cols, q, enable_ordering=True, v_begin=0, v_end=None, threads=0) -> int:
+    rp, cols = _arrays(rp, cols)
+    n = rp.size - 1
+    out = ctypes.c_uint64()
+    rc = lib().or_wcoj_count(n, rp.ctypes.data, cols.ctypes.data, q, int(enable_ordering), v_begin,
+                             n if v_end is None else v_end, threads, ctypes.byref(out))
+    if rc:
+        raise ValueError(rc)
+    return int(out.value)
+
+
+def light_count(rp, cols, q, compress_level=3, enable_ordering=True, v_begin=0, v_end=None,
+                threads=0):
+    """Returns (count, per-exec-level outputs)."""
+    rp, cols = _arrays(rp, cols)
+    n = rp.size - 1
+    out = ctypes.c_uint64()
+    lv = (ctypes.c_uint64 * MAX_OPS)()
+    rc = lib().or_light_count(n, rp.ctypes.data, cols.ctypes.data, q, compress_level,
+                              int(enable_ordering), v_begin, n if v_end is None else v_end, threads,
+                              ctypes.byref(out), lv)
+    if rc:
+        raise ValueError(rc)
+    nops = plan_text(q, compress_level, enable_ordering).split("\n")[1].count(" ")
+    return int(out.value), [int(lv[i]) for i in range(nops)]
+
+
+def gpsm_count(rp, cols, q, enable_ordering=True, v_begin=0, v_end=None, threads=0):
+    rp, cols = _arrays(rp, cols)
+    n = rp.size - 1
+    out = ctypes.c_uint64()
+    lv = (ctypes.c_uint64 * MAX_OPS)()
+    rc = lib().or_gpsm_count(n, rp.ctypes.data, cols.ctypes.data, q, int(enable_ordering), v_begin,
+                             n if v_end is None else v_end, threads, ctypes.byref(out), lv)
+    if rc:
+        raise ValueError(rc)
+    return int(out.value), [int(lv[i]) for i in range(MAX_OPS)]
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_CPU)
+
+
+def ref_count(bin_path: str, q: int, algo: int = 0, threads: int = 1, buffer_gb: int = 1,
+              timeout: float = 600.0):
+    """Runs the reference CPU binary (built from the reference sources into
+    oracle/_ref) and returns (total_match_count, elapsed_ms)."""
+    cmd = [REF_CPU, "-f", bin_path, "-d", "0", "-a", str(algo), "-q", str(q), "-t", str(threads),
+           "-b", str(buffer_gb)]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout)
+    for line in r.stdout.splitlines():
+        if line.startswith("total_match_count="):
+            a, b = line.split(",")
+            return int(a.split("=")[1]), float(b.split("=")[1].rstrip("ms"))
+    raise RuntimeError(f"reference run failed: rc={r.returncode} {r.stdout[-400:]} {r.stderr[-400:]}")

@@ -1,0 +1,316 @@
+"""MI355X subgraph enumeration engine: Python view of the C ABI in include/sme.h.
+
+The compute path is libsme.so (hand-written gfx950 HIP kernels behind an
+extern "C" boundary).  This module only loads it with ctypes and moves numpy
+arrays across; there is no Python or CPU fallback.  If the library is missing
+every entry point raises.
+
+Mirrors the reference's host interface for the GPU path:
+  Query ids / QueryType              src/Meta.h:104-124
+  Algo ids (GPU_GPSM=4, GPU_LIGHT=6) src/Meta.h:47-66
+  compress levels                    src/query/common/LazyTraversalCommon.h:38-76
+  count over a start-vertex range    HybridGPUComponent::GPUThreadExecute
+                                     (src/gpu/impl/common/HybridGPUComp.cuh:53-69)
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsme.so")
+CLI_PATH = os.path.join(_HERE, "patternmatchgpu")
+
+# QueryType (src/Meta.h:104-124)
+Q0, Q1, Q2, Q3, Q4, Q5, Q6, Q7, Q8, Q9, Q10, Q11, Q12, Q13 = range(14)
+QUERY_NAMES = {
+    0: "triangle", 1: "square", 2: "chordal_square", 3: "4clique", 4: "house",
+    5: "quad_triangle", 6: "near5clique", 7: "5clique", 8: "chordal_roof",
+    9: "three_triangle", 10: "solar_square", 11: "6clique", 12: "near_near_5clique",
+    13: "triangle_in_diamond",
+}
+# Algo (src/Meta.h:47-66)
+GPU_GPSM, GPU_GPSM_REUSE, GPU_LIGHT = 4, 5, 6
+# LazyTraversalCompressLevel
+CL_MATERIALIZE, CL_NON_MATERIALIZE, CL_NON_MATERIALIZE_OPT, CL_SPECIAL = range(4)
+
+STATUS = {0: "OK", 1: "INVALID", 2: "OOM", 3: "HIP", 4: "IO", 5: "UNSUPPORTED"}
+MAX_LEVELS = 16
+
+
+class SmeError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"{STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+class _Stats(ctypes.Structure):
+    _fields_ = [("batches", ctypes.c_uint64), ("d2h_syncs", ctypes.c_uint64),
+                ("peak_pool_bytes", ctypes.c_uint64), ("elapsed_ms", ctypes.c_double)]
+
+
+_lib = None
+
+
+def library() -> ctypes.CDLL:
+    """Loads libsme.so (built in-tree by __graft_entry__.build / make)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built; run `make -C subgraphmatchgpu_amd`")
+    lib = ctypes.CDLL(LIB_PATH)
+    u64p = ctypes.POINTER(ctypes.c_uint64)
+    u32p = ctypes.POINTER(ctypes.c_uint32)
+    vp = ctypes.c_void_p
+    sig = {
+        "sme_open": ([ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(vp)], ctypes.c_int),
+        "sme_close": ([vp], None),
+        "sme_last_error": ([vp], ctypes.c_char_p),
+        "sme_device_count": ([ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+        "sme_graph_upload": ([vp, vp, vp, ctypes.c_uint64, ctypes.c_uint64, ctypes.POINTER(vp)],
+                             ctypes.c_int),
+        "sme_graph_destroy": ([vp], None),
+        "sme_plan_create": ([ctypes.c_int] * 4 + [ctypes.POINTER(vp)], ctypes.c_int),
+        "sme_plan_describe": ([vp, ctypes.c_char_p, ctypes.c_size_t], ctypes.c_int),
+        "sme_plan_levels": ([vp], ctypes.c_int),
+        "sme_plan_destroy": ([vp], None),
+        "sme_count": ([vp, vp, vp, ctypes.c_uint32, ctypes.c_uint32, u64p, u64p,
+                       ctypes.POINTER(_Stats)], ctypes.c_int),
+        "sme_profile_enable": ([vp, ctypes.c_int], ctypes.c_int),
+        "sme_profile_reset": ([vp], ctypes.c_int),
+        "sme_profile_slots": ([], ctypes.c_int),
+        "sme_profile_get": ([vp, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t, u64p,
+                             ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), u64p],
+                            ctypes.c_int),
+        "sme_bin_read": ([ctypes.c_char_p, u64p, u64p, ctypes.POINTER(u64p), ctypes.POINTER(u32p)],
+                         ctypes.c_int),
+        "sme_bin_write": ([ctypes.c_char_p, vp, vp, ctypes.c_uint64, ctypes.c_uint64], ctypes.c_int),
+        "sme_snap_read": ([ctypes.c_char_p, ctypes.c_int, u64p, u64p, ctypes.POINTER(u64p),
+                           ctypes.POINTER(u32p)], ctypes.c_int),
+        "sme_gen_rmat": ([ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                          ctypes.c_double, ctypes.c_uint64, u64p, u64p, ctypes.POINTER(u64p),
+                          ctypes.POINTER(u32p)], ctypes.c_int),
+        "sme_gen_er": ([ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, u64p, u64p,
+                        ctypes.POINTER(u64p), ctypes.POINTER(u32p)], ctypes.c_int),
+        "sme_free_host": ([vp], None),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = lib
+    return lib
+
+
+def _check(rc: int, ctx=None):
+    if rc != 0:
+        msg = library().sme_last_error(ctx).decode(errors="replace")
+        raise SmeError(rc, msg)
+
+
+@dataclass
+class Graph:
+    """Host CSR in the reference's BIN layout (src/graph/GraphIO.h:104-127)."""
+    row_ptrs: np.ndarray  # uint64[n+1]
+    cols: np.ndarray      # uint32[nnz]
+
+    @property
+    def n(self) -> int:
+        return int(self.row_ptrs.shape[0] - 1)
+
+    @property
+    def nnz(self) -> int:
+        return int(self.cols.shape[0])
+
+    def checksum(self) -> str:
+        import hashlib
+        h = hashlib.sha256()
+        h.update(np.ascontiguousarray(self.row_ptrs, dtype=np.uint64).tobytes())
+        h.update(np.ascontiguousarray(self.cols, dtype=np.uint32).tobytes())
+        return h.hexdigest()
+
+
+def _take_host(n, nnz, rp, cols) -> Graph:
+    lib = library()
+    a = np.ctypeslib.as_array(rp, shape=(n.value + 1,)).copy()
+    b = (np.ctypeslib.as_array(cols, shape=(nnz.value,)).copy() if nnz.value
+         else np.zeros(0, dtype=np.uint32))
+    lib.sme_free_host(ctypes.cast(rp, ctypes.c_void_p))
+    lib.sme_free_host(ctypes.cast(cols, ctypes.c_void_p))
+    return Graph(a.astype(np.uint64, copy=False), b.astype(np.uint32, copy=False))
+
+
+def _out_args():
+    return (ctypes.c_uint64(), ctypes.c_uint64(), ctypes.POINTER(ctypes.c_uint64)(),
+            ctypes.POINTER(ctypes.c_uint32)())
+
+
+def gen_rmat(scale: int, edge_factor: int, a=0.57, b=0.19, c=0.19, seed=1) -> Graph:
+    """Seeded R-MAT, symmetrized, self loops and duplicates removed."""
+    n, nnz, rp, cols = _out_args()
+    _check(library().sme_gen_rmat(scale, edge_factor, a, b, c, seed, ctypes.byref(n),
+                                  ctypes.byref(nnz), ctypes.byref(rp), ctypes.byref(cols)))
+    return _take_host(n, nnz, rp, cols)
+
+
+def gen_er(n_vertices: int, n_edges: int, seed=1) -> Graph:
+    """Seeded G(n, m) with m distinct undirected edges."""
+    n, nnz, rp, cols = _out_args()
+    _check(library().sme_gen_er(n_vertices, n_edges, seed, ctypes.byref(n), ctypes.byref(nnz),
+                                ctypes.byref(rp), ctypes.byref(cols)))
+    return _take_host(n, nnz, rp, cols)
+
+
+def read_bin(path: str) -> Graph:
+    n, nnz, rp, cols = _out_args()
+    _check(library().sme_bin_read(path.encode(), ctypes.byref(n), ctypes.byref(nnz),
+                                  ctypes.byref(rp), ctypes.byref(cols)))
+    return _take_host(n, nnz, rp, cols)
+
+
+def write_bin(path: str, g: Graph) -> None:
+    rp = np.ascontiguousarray(g.row_ptrs, dtype=np.uint64)
+    cols = np.ascontiguousarray(g.cols, dtype=np.uint32)
+    _check(library().sme_bin_write(path.encode(), rp.ctypes.data, cols.ctypes.data, g.n, g.nnz))
+
+
+def read_snap(path: str, directed: bool = False) -> Graph:
+    n, nnz, rp, cols = _out_args()
+    _check(library().sme_snap_read(path.encode(), int(directed), ctypes.byref(n), ctypes.byref(nnz),
+                                   ctypes.byref(rp), ctypes.byref(cols)))
+    return _take_host(n, nnz, rp, cols)
+
+
+def from_adjacency(adj) -> Graph:
+    """CSR from a list of neighbour lists (Graph(vector<vector<uintV>>),
+    src/graph/CPUGraph.h:16-18); lists are sorted here."""
+    rp = np.zeros(len(adj) + 1, dtype=np.uint64)
+    for i, lst in enumerate(adj):
+        rp[i + 1] = rp[i] + len(lst)
+    cols = np.concatenate([np.sort(np.asarray(l, dtype=np.uint32)) for l in adj]) if len(adj) else \
+        np.zeros(0, dtype=np.uint32)
+    return Graph(rp, cols.astype(np.uint32))
+
+
+class Plan:
+    def __init__(self, query: int, algo: int = GPU_LIGHT, compress_level: int = CL_SPECIAL,
+                 enable_ordering: bool = True):
+        lib = library()
+        h = ctypes.c_void_p()
+        _check(lib.sme_plan_create(algo, query, compress_level, int(enable_ordering), ctypes.byref(h)))
+        self._h = h
+        self.query, self.algo, self.compress_level = query, algo, compress_level
+        self.enable_ordering = enable_ordering
+
+    @property
+    def levels(self) -> int:
+        return library().sme_plan_levels(self._h)
+
+    def describe(self) -> str:
+        lib = library()
+        n = lib.sme_plan_describe(self._h, None, 0)
+        buf = ctypes.create_string_buffer(n + 1)
+        lib.sme_plan_describe(self._h, buf, n + 1)
+        return buf.value.decode()
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.sme_plan_destroy(self._h)
+            self._h = None
+
+
+@dataclass
+class CountResult:
+    count: int
+    levels: list
+    batches: int
+    d2h_syncs: int
+    peak_pool_bytes: int
+    elapsed_ms: float
+
+
+class Device:
+    """One GPU: a stream plus a memory pool of `budget_bytes` for intermediates
+    (the reference's per-device limit, src/Meta.h:26-28)."""
+
+    def __init__(self, device: int = 0, budget_bytes: int = 8 << 30):
+        lib = library()
+        h = ctypes.c_void_p()
+        _check(lib.sme_open(device, budget_bytes, ctypes.byref(h)))
+        self._h = h
+        self._graphs = []
+
+    def upload(self, g: Graph) -> "DeviceGraph":
+        return DeviceGraph(self, g)
+
+    def count(self, dg: "DeviceGraph", plan: Plan, v_begin: int = 0, v_end: int | None = None) -> CountResult:
+        lib = library()
+        if v_end is None:
+            v_end = dg.n
+        cnt = ctypes.c_uint64()
+        lv = (ctypes.c_uint64 * MAX_LEVELS)()
+        st = _Stats()
+        _check(lib.sme_count(self._h, dg._h, plan._h, v_begin, v_end, ctypes.byref(cnt), lv,
+                             ctypes.byref(st)), self._h)
+        return CountResult(int(cnt.value), [int(lv[i]) for i in range(plan.levels)], int(st.batches),
+                           int(st.d2h_syncs), int(st.peak_pool_bytes), float(st.elapsed_ms))
+
+    def profile(self, on: bool = True):
+        _check(library().sme_profile_enable(self._h, int(on)), self._h)
+
+    def profile_reset(self):
+        _check(library().sme_profile_reset(self._h), self._h)
+
+    def profile_read(self) -> dict:
+        lib = library()
+        out = {}
+        for s in range(lib.sme_profile_slots()):
+            name = ctypes.create_string_buffer(64)
+            la, ms, by, it = ctypes.c_uint64(), ctypes.c_double(), ctypes.c_double(), ctypes.c_uint64()
+            _check(lib.sme_profile_get(self._h, s, name, 64, ctypes.byref(la), ctypes.byref(ms),
+                                       ctypes.byref(by), ctypes.byref(it)), self._h)
+            out[name.value.decode()] = {"launches": la.value, "ms": ms.value, "bytes": by.value,
+                                        "items": it.value}
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None):
+            for g in self._graphs:
+                g.release()
+            library().sme_close(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceGraph:
+    def __init__(self, dev: Device, g: Graph):
+        lib = library()
+        self._rp = np.ascontiguousarray(g.row_ptrs, dtype=np.uint64)
+        self._cols = np.ascontiguousarray(g.cols, dtype=np.uint32)
+        h = ctypes.c_void_p()
+        _check(lib.sme_graph_upload(dev._h, self._rp.ctypes.data,
+                                    self._cols.ctypes.data if g.nnz else None, g.n, g.nnz,
+                                    ctypes.byref(h)), dev._h)
+        self._h = h
+        self.n, self.nnz = g.n, g.nnz
+        dev._graphs.append(self)
+
+    def release(self):
+        if getattr(self, "_h", None):
+            library().sme_graph_destroy(self._h)
+            self._h = None
+
+
+def device_count() -> int:
+    c = ctypes.c_int()
+    _check(library().sme_device_count(ctypes.byref(c)))
+    return c.value

@@ -1,0 +1,258 @@
+// capi.cpp -- the extern "C" boundary declared in include/sme.h.
+#include <hip/hip_runtime.h>
+
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "../../include/sme.h"
+#include "engine.h"
+#include "graph.h"
+#include "query.h"
+
+struct sme_ctx {
+  sme::Context impl;
+};
+struct sme_graph {
+  sme::DevGraph impl;
+};
+struct sme_plan {
+  sme::EnginePlan plan;
+  std::string text;
+};
+
+namespace {
+
+thread_local std::string g_last_error;  // errors without a context
+
+int Report(sme_ctx* ctx, int status, const std::string& msg) {
+  if (status != SME_OK) {
+    if (ctx) ctx->impl.last_error = msg;
+    g_last_error = msg;
+  }
+  return status;
+}
+
+template <typename T>
+T* CopyOut(const std::vector<T>& v) {
+  T* p = static_cast<T*>(std::malloc(v.empty() ? sizeof(T) : v.size() * sizeof(T)));
+  if (p && !v.empty()) std::memcpy(p, v.data(), v.size() * sizeof(T));
+  return p;
+}
+
+int HandOut(const sme::HostGraph& g, uint64_t* n, uint64_t* nnz, uint64_t** rp, uint32_t** cols) {
+  *n = g.n;
+  *nnz = g.nnz;
+  *rp = CopyOut(g.row_ptrs);
+  *cols = CopyOut(g.cols);
+  if (!*rp || !*cols) {
+    std::free(*rp);
+    std::free(*cols);
+    *rp = nullptr;
+    *cols = nullptr;
+    return Report(nullptr, SME_ERR_OOM, "host allocation failed");
+  }
+  return SME_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sme_device_count(int* out) {
+  if (!out) return SME_ERR_INVALID;
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess) {
+    (void)hipGetLastError();
+    c = 0;
+  }
+  *out = c;
+  return SME_OK;
+}
+
+int sme_open(int device, uint64_t mem_budget_bytes, sme_ctx** out) {
+  if (!out) return Report(nullptr, SME_ERR_INVALID, "null output pointer");
+  *out = nullptr;
+  sme_ctx* c = new (std::nothrow) sme_ctx;
+  if (!c) return Report(nullptr, SME_ERR_OOM, "host allocation failed");
+  std::string err;
+  const sme::Status s = c->impl.Open(device, mem_budget_bytes, &err);
+  if (s != sme::kOk) {
+    delete c;
+    return Report(nullptr, s, err);
+  }
+  *out = c;
+  return SME_OK;
+}
+
+void sme_close(sme_ctx* ctx) { delete ctx; }
+
+const char* sme_last_error(const sme_ctx* ctx) {
+  if (ctx && !ctx->impl.last_error.empty()) return ctx->impl.last_error.c_str();
+  return g_last_error.c_str();
+}
+
+int sme_graph_upload(sme_ctx* ctx, const uint64_t* row_ptrs, const uint32_t* cols, uint64_t n,
+                     uint64_t nnz, sme_graph** out) {
+  if (!ctx || !out) return Report(ctx, SME_ERR_INVALID, "null argument");
+  *out = nullptr;
+  sme_graph* g = new (std::nothrow) sme_graph;
+  if (!g) return Report(ctx, SME_ERR_OOM, "host allocation failed");
+  std::string err;
+  sme::Status s;
+  try {
+    s = g->impl.Upload(&ctx->impl, row_ptrs, cols, n, nnz, &err);
+  } catch (const sme::Error& e) {
+    s = e.status;
+    err = e.msg;
+  }
+  if (s != sme::kOk) {
+    delete g;
+    return Report(ctx, s, err);
+  }
+  *out = g;
+  return SME_OK;
+}
+
+void sme_graph_destroy(sme_graph* g) { delete g; }
+
+int sme_plan_create(int algo, int query_type, int compress_level, int enable_ordering,
+                    sme_plan** out) {
+  if (!out) return Report(nullptr, SME_ERR_INVALID, "null output pointer");
+  *out = nullptr;
+  sme_plan* p = new (std::nothrow) sme_plan;
+  if (!p) return Report(nullptr, SME_ERR_OOM, "host allocation failed");
+  bool ok = false;
+  if (algo == SME_ALGO_LIGHT) {
+    ok = sme::MakeLightEnginePlan(query_type, compress_level, enable_ordering != 0, &p->plan, &p->text);
+  } else if (algo == SME_ALGO_GPSM) {
+    ok = sme::MakeGpsmEnginePlan(query_type, enable_ordering != 0, &p->plan, &p->text);
+  } else {
+    delete p;
+    return Report(nullptr, SME_ERR_UNSUPPORTED, "algorithm " + std::to_string(algo) + " is not supported");
+  }
+  if (!ok) {
+    delete p;
+    return Report(nullptr, SME_ERR_INVALID,
+                  "cannot build a plan for query " + std::to_string(query_type) + " (compress level " +
+                      std::to_string(compress_level) + ")");
+  }
+  *out = p;
+  return SME_OK;
+}
+
+int sme_plan_describe(const sme_plan* p, char* buf, size_t cap) {
+  if (!p) return -1;
+  if (buf && cap) {
+    const size_t n = p->text.size() < cap - 1 ? p->text.size() : cap - 1;
+    std::memcpy(buf, p->text.data(), n);
+    buf[n] = '\0';
+  }
+  return static_cast<int>(p->text.size());
+}
+
+int sme_plan_levels(const sme_plan* p) { return p ? p->plan.nops : -1; }
+
+void sme_plan_destroy(sme_plan* p) { delete p; }
+
+int sme_count(sme_ctx* ctx, const sme_graph* g, const sme_plan* p, uint32_t v_begin, uint32_t v_end,
+              uint64_t* out_count, uint64_t* level_counts, sme_stats* stats) {
+  if (!ctx || !g || !p || !out_count) return Report(ctx, SME_ERR_INVALID, "null argument");
+  std::string err;
+  sme::RunStats rs;
+  const sme::Status s =
+      sme::RunPlan(&ctx->impl, &g->impl, &p->plan, v_begin, v_end, out_count, level_counts, &rs, &err);
+  if (s != sme::kOk) return Report(ctx, s, err);
+  if (stats) {
+    stats->batches = rs.batches;
+    stats->d2h_syncs = rs.d2h_syncs;
+    stats->peak_pool_bytes = rs.peak_pool_bytes;
+    stats->elapsed_ms = rs.elapsed_ms;
+  }
+  return SME_OK;
+}
+
+int sme_profile_enable(sme_ctx* ctx, int on) {
+  if (!ctx) return SME_ERR_INVALID;
+  ctx->impl.profiling = on != 0;
+  return SME_OK;
+}
+
+int sme_profile_reset(sme_ctx* ctx) {
+  if (!ctx) return SME_ERR_INVALID;
+  ctx->impl.ResetProfile();
+  return SME_OK;
+}
+
+int sme_profile_slots(void) { return sme::kNumSlots; }
+
+int sme_profile_get(const sme_ctx* ctx, int slot, char* name, size_t cap, uint64_t* launches, double* ms,
+                    double* bytes, uint64_t* items) {
+  if (!ctx || slot < 0 || slot >= sme::kNumSlots) return SME_ERR_INVALID;
+  const sme::SlotStats& s = ctx->impl.slots[slot];
+  if (name && cap) {
+    std::strncpy(name, sme::SlotName(slot), cap - 1);
+    name[cap - 1] = '\0';
+  }
+  if (launches) *launches = s.launches;
+  if (ms) *ms = s.ms;
+  if (bytes) *bytes = s.bytes;
+  if (items) *items = s.items;
+  return SME_OK;
+}
+
+int sme_bin_read(const char* path, uint64_t* n, uint64_t* nnz, uint64_t** row_ptrs, uint32_t** cols) {
+  if (!path || !n || !nnz || !row_ptrs || !cols) return Report(nullptr, SME_ERR_INVALID, "null argument");
+  sme::HostGraph g;
+  std::string err;
+  const sme::Status s = sme::ReadBin(path, &g, &err);
+  if (s != sme::kOk) return Report(nullptr, s, err);
+  return HandOut(g, n, nnz, row_ptrs, cols);
+}
+
+int sme_bin_write(const char* path, const uint64_t* row_ptrs, const uint32_t* cols, uint64_t n,
+                  uint64_t nnz) {
+  if (!path || !row_ptrs || (nnz && !cols)) return Report(nullptr, SME_ERR_INVALID, "null argument");
+  sme::HostGraph g;
+  g.n = n;
+  g.nnz = nnz;
+  g.row_ptrs.assign(row_ptrs, row_ptrs + n + 1);
+  g.cols.assign(cols, cols + nnz);
+  std::string err;
+  const sme::Status s = sme::WriteBin(path, g, &err);
+  return Report(nullptr, s, err);
+}
+
+int sme_snap_read(const char* path, int directed, uint64_t* n, uint64_t* nnz, uint64_t** row_ptrs,
+                  uint32_t** cols) {
+  if (!path || !n || !nnz || !row_ptrs || !cols) return Report(nullptr, SME_ERR_INVALID, "null argument");
+  sme::HostGraph g;
+  std::string err;
+  const sme::Status s = sme::ReadSnap(path, directed != 0, &g, &err);
+  if (s != sme::kOk) return Report(nullptr, s, err);
+  return HandOut(g, n, nnz, row_ptrs, cols);
+}
+
+int sme_gen_rmat(int scale, int edge_factor, double a, double b, double c, uint64_t seed, uint64_t* n,
+                 uint64_t* nnz, uint64_t** row_ptrs, uint32_t** cols) {
+  if (scale < 1 || scale > 31 || edge_factor < 1 || a < 0 || b < 0 || c < 0 || a + b + c > 1.0)
+    return Report(nullptr, SME_ERR_INVALID, "bad R-MAT parameters");
+  if (!n || !nnz || !row_ptrs || !cols) return Report(nullptr, SME_ERR_INVALID, "null argument");
+  sme::HostGraph g;
+  sme::GenerateRmat(scale, edge_factor, a, b, c, seed, &g);
+  return HandOut(g, n, nnz, row_ptrs, cols);
+}
+
+int sme_gen_er(uint64_t n_vertices, uint64_t n_edges, uint64_t seed, uint64_t* n, uint64_t* nnz,
+               uint64_t** row_ptrs, uint32_t** cols) {
+  if (n_vertices < 2 || n_vertices >= 0xffffffffull) return Report(nullptr, SME_ERR_INVALID, "bad vertex count");
+  if (!n || !nnz || !row_ptrs || !cols) return Report(nullptr, SME_ERR_INVALID, "null argument");
+  sme::HostGraph g;
+  sme::GenerateErdosRenyi(n_vertices, n_edges, seed, &g);
+  return HandOut(g, n, nnz, row_ptrs, cols);
+}
+
+void sme_free_host(void* p) { std::free(p); }
+
+}  // extern "C"

@@ -1,0 +1,107 @@
+// engine.h -- per-device context, resident graph and the level/batch engine.
+//
+// Context  ~ CudaContext + CudaContextManager (src/gpu/utils/context/
+//            CudaContext.cuh:83-98, CudaContextManager.cuh:19-103): one stream
+//            and one DevicePool per device.
+// DevGraph ~ DevGraphPartition (src/gpu/graph/DevGraphPartition.cuh:32-41).
+// Engine   ~ HybridGPUProcessor::ProcessLevel + HybridLIGHTGPUProcessor
+//            (src/gpu/impl/common/HybridGPUProcessor.cuh:14-47,
+//            src/gpu/impl/light/HybridLIGHTGPUProcessor.cuh:22-316): a
+//            depth-first walk over memory-capped batches of each BFS level.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "kernels.h"
+#include "pool.h"
+#include "query.h"
+
+namespace sme {
+
+// Kernel families timed with HIP events when profiling is on.
+enum KernelSlot : int {
+  kSlotPrep = 0,
+  kSlotScan,
+  kSlotPartition,
+  kSlotIntersect,   // item verify kernels of COMPUTE / CPC / COMPUTE_COUNT
+  kSlotFilter,      // item verify kernels of MATERIALIZE / FILTER_COMPUTE
+  kSlotCount,       // COUNT kernels
+  kSlotCompact,     // candidate compaction + segment windows
+  kSlotExpand,      // MATERIALIZE scatter of partial instances
+  kSlotMisc,
+  kNumSlots
+};
+const char* SlotName(int slot);
+
+struct SlotStats {
+  uint64_t launches = 0;
+  double ms = 0;
+  double bytes = 0;  // algorithmic bytes
+  uint64_t items = 0;
+};
+
+struct RunStats {
+  uint64_t batches = 0;
+  uint64_t d2h_syncs = 0;
+  uint64_t peak_pool_bytes = 0;
+  double elapsed_ms = 0;
+};
+
+class Context {
+ public:
+  ~Context();
+  Status Open(int device, uint64_t budget_bytes, std::string* err);
+  int device() const { return device_; }
+  hipStream_t stream() const { return stream_; }
+  DevicePool& pool() { return pool_; }
+  uint64_t* pinned() { return pinned_; }
+
+  // profiling
+  bool profiling = false;
+  std::array<SlotStats, kNumSlots> slots{};
+  void ResetProfile() { slots = {}; }
+  std::string last_error;
+
+  // event pairs recorded during a run, resolved after the final sync
+  struct Mark {
+    hipEvent_t start, stop;
+    int slot;
+    double bytes;
+    uint64_t items;
+  };
+  std::vector<Mark> marks;
+  std::vector<hipEvent_t> spare_events;
+  hipEvent_t TakeEvent();
+  void ResolveMarks();
+
+ private:
+  int device_ = -1;
+  hipStream_t stream_ = nullptr;
+  DevicePool pool_;
+  uint64_t* pinned_ = nullptr;
+};
+
+class DevGraph {
+ public:
+  ~DevGraph();
+  Status Upload(Context* ctx, const uint64_t* row_ptrs, const uint32_t* cols, uint64_t n,
+                uint64_t nnz, std::string* err);
+  uint64_t n = 0, nnz = 0;
+  uint64_t* rp = nullptr;
+  uint32_t* cols = nullptr;
+  bool self_loops = false;
+  int device = -1;
+};
+
+// Runs one plan over start vertices [v_begin, v_end) of the first matched
+// vertex (an IntraPartTask, src/gpu/task/Task.h:23-43).
+Status RunPlan(Context* ctx, const DevGraph* g, const EnginePlan* plan, uint32_t v_begin,
+               uint32_t v_end, uint64_t* count, uint64_t* level_counts, RunStats* stats,
+               std::string* err);
+
+}  // namespace sme

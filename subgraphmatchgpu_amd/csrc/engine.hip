@@ -1,0 +1,880 @@
+// engine.hip -- context, resident graph and the level/batch engine (engine.h).
+//
+// Host orchestration only; every device step is a kernel from kernels.hip.
+#include "engine.h"
+
+#include <algorithm>
+#include <chrono>
+#include <memory>
+#include <cstring>
+#include <utility>
+
+#include "graph.h"
+
+namespace sme {
+
+void Fail(Status s, const std::string& msg) { throw Error{s, msg}; }
+
+#define SME_HIP(expr)                                                            \
+  do {                                                                           \
+    hipError_t e_ = (expr);                                                      \
+    if (e_ != hipSuccess) Fail(kErrHip, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+const char* SlotName(int slot) {
+  static const char* kNames[kNumSlots] = {"prep",  "scan",    "partition", "intersect", "filter",
+                                          "count", "compact", "expand",    "misc"};
+  return slot >= 0 && slot < kNumSlots ? kNames[slot] : "?";
+}
+
+// ---------------------------------------------------------------------------
+Context::~Context() {
+  for (auto& m : marks) {
+    (void)hipEventDestroy(m.start);
+    (void)hipEventDestroy(m.stop);
+  }
+  for (auto e : spare_events) (void)hipEventDestroy(e);
+  pool_.Release();
+  if (pinned_) (void)hipHostFree(pinned_);
+  if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+Status Context::Open(int device, uint64_t budget_bytes, std::string* err) {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+    (void)hipGetLastError();
+    *err = "no HIP device is visible";
+    return kErrHip;
+  }
+  if (device < 0 || device >= count) {
+    *err = "device id out of range";
+    return kErrInvalid;
+  }
+  device_ = device;
+  if (hipSetDevice(device) != hipSuccess) {
+    *err = "hipSetDevice failed";
+    return kErrHip;
+  }
+  // default-priority stream
+  if (hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking) != hipSuccess) {
+    *err = "hipStreamCreate failed";
+    return kErrHip;
+  }
+  if (hipHostMalloc(reinterpret_cast<void**>(&pinned_), 4096, hipHostMallocDefault) != hipSuccess) {
+    *err = "hipHostMalloc failed";
+    return kErrHip;
+  }
+  if (!pool_.Init(budget_bytes)) {
+    *err = "cannot reserve the device pool of " + std::to_string(budget_bytes) + " bytes";
+    return kErrOom;
+  }
+  return kOk;
+}
+
+hipEvent_t Context::TakeEvent() {
+  if (!spare_events.empty()) {
+    hipEvent_t e = spare_events.back();
+    spare_events.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  SME_HIP(hipEventCreate(&e));
+  return e;
+}
+
+void Context::ResolveMarks() {
+  for (auto& m : marks) {
+    float ms = 0;
+    SME_HIP(hipEventElapsedTime(&ms, m.start, m.stop));
+    SlotStats& s = slots[m.slot];
+    s.launches++;
+    s.ms += ms;
+    s.bytes += m.bytes;
+    s.items += m.items;
+    spare_events.push_back(m.start);
+    spare_events.push_back(m.stop);
+  }
+  marks.clear();
+}
+
+// ---------------------------------------------------------------------------
+DevGraph::~DevGraph() {
+  if (device >= 0) (void)hipSetDevice(device);
+  if (rp) (void)hipFree(rp);
+  if (cols) (void)hipFree(cols);
+}
+
+Status DevGraph::Upload(Context* ctx, const uint64_t* row_ptrs, const uint32_t* c, uint64_t nv,
+                        uint64_t ne, std::string* err) {
+  bool loops = false;
+  if (!row_ptrs || (ne && !c)) {
+    *err = "null graph arrays";
+    return kErrInvalid;
+  }
+  if (nv >= 0xffffffffull) {
+    *err = "vertex ids must fit in 32 bits";
+    return kErrInvalid;
+  }
+  if (!CheckCsr(nv, ne, row_ptrs, c, &loops, err)) return kErrInvalid;
+  device = ctx->device();
+  SME_HIP(hipSetDevice(device));
+  n = nv;
+  nnz = ne;
+  self_loops = loops;
+  if (hipMalloc(reinterpret_cast<void**>(&rp), (nv + 1) * sizeof(uint64_t)) != hipSuccess ||
+      hipMalloc(reinterpret_cast<void**>(&cols), (ne ? ne : 1) * sizeof(uint32_t)) != hipSuccess) {
+    (void)hipGetLastError();
+    *err = "hipMalloc of the CSR failed";
+    return kErrOom;
+  }
+  SME_HIP(hipMemcpyAsync(rp, row_ptrs, (nv + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, ctx->stream()));
+  if (ne)
+    SME_HIP(hipMemcpyAsync(cols, c, ne * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream()));
+  SME_HIP(hipStreamSynchronize(ctx->stream()));
+  return kOk;
+}
+
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr uint64_t kMaxBatchItems = 1ull << 31;
+constexpr uint64_t kMaxBatchPaths = 1ull << 31;
+
+// Pool-backed device array, released when it leaves scope.
+template <typename T>
+class Buf {
+ public:
+  Buf(DevicePool* pool, uint64_t n) : pool_(pool), n_(n) {
+    if (n == 0) return;
+    p_ = static_cast<T*>(pool->Alloc(n * sizeof(T)));
+    if (!p_)
+      Fail(kErrOom, "device pool exhausted: need " + std::to_string(n * sizeof(T)) + " bytes, " +
+                        std::to_string(pool->Available()) + " available");
+  }
+  ~Buf() { Reset(); }
+  void Reset() {
+    if (p_) pool_->Free(p_);
+    p_ = nullptr;
+  }
+  Buf(const Buf&) = delete;
+  Buf& operator=(const Buf&) = delete;
+  T* get() const { return p_; }
+  uint64_t size() const { return n_; }
+
+ private:
+  DevicePool* pool_;
+  uint64_t n_;
+  T* p_ = nullptr;
+};
+
+struct Frame {
+  uint64_t P = 0;
+  ColSet cs{};
+  uint32_t matmask = 0;
+  CandRef cand[kQ]{};
+  uint32_t candmask = 0;
+};
+
+ColSet SliceCols(const ColSet& c, uint64_t l) {
+  ColSet r = c;
+  for (int w = 0; w < kQ; ++w) {
+    if (r.inst[w])
+      r.inst[w] += l;
+    else
+      r.iota[w] += static_cast<uint32_t>(l);
+  }
+  return r;
+}
+
+CandRef SliceCand(const CandRef& c, uint64_t l) {
+  CandRef r = c;
+  if (r.idx) {
+    r.idx += l;
+  } else {
+    if (r.beg) r.beg += l;
+    if (r.len) r.len += l;
+  }
+  return r;
+}
+
+Frame SliceFrame(const Frame& f, uint64_t l, uint64_t r) {
+  Frame s = f;
+  s.P = r - l;
+  s.cs = SliceCols(f.cs, l);
+  for (int w = 0; w < kQ; ++w)
+    if (f.candmask & (1u << w)) s.cand[w] = SliceCand(f.cand[w], l);
+  return s;
+}
+
+class Runner {
+ public:
+  Runner(Context* ctx, const DevGraph* g, const EnginePlan* plan)
+      : ctx_(ctx), g_(g), plan_(plan), st_(ctx->stream()), pool_(&ctx->pool()) {}
+
+  void Run(uint32_t v_begin, uint32_t v_end, uint64_t* count, uint64_t* levels, RunStats* stats) {
+    Analyze();
+    Buf<unsigned long long> acc(pool_, 1 + plan_->nops);
+    acc_ = acc.get();
+    SME_HIP(hipMemsetAsync(acc_, 0, sizeof(unsigned long long) * (1 + plan_->nops), st_));
+    std::memset(host_level_, 0, sizeof(host_level_));
+    host_count_ = 0;
+    if (v_end > v_begin) Level0(v_begin, v_end);
+    unsigned long long dev[1 + kMaxOps];
+    SME_HIP(hipMemcpyAsync(ctx_->pinned(), acc_, sizeof(unsigned long long) * (1 + plan_->nops),
+                           hipMemcpyDeviceToHost, st_));
+    SME_HIP(hipStreamSynchronize(st_));
+    std::memcpy(dev, ctx_->pinned(), sizeof(unsigned long long) * (1 + plan_->nops));
+    stats_.d2h_syncs++;
+    uint64_t total = host_count_ + dev[0];
+    for (int L = 0; L < plan_->nops; ++L) levels_[L] = host_level_[L] + dev[1 + L];
+    if (plan_->algo == kAlgoGpsm) total = levels_[plan_->nops - 1];
+    *count = total;
+    if (levels)
+      for (int L = 0; L < plan_->nops; ++L) levels[L] = levels_[L];
+    if (stats) *stats = stats_;
+  }
+
+ private:
+  // ----- plan analysis -----
+  void Analyze() {
+    const int n = plan_->nops;
+    uint32_t need_after = 0;
+    for (int L = n - 1; L >= 0; --L) {
+      live_after_[L] = need_after;
+      const OpDesc& op = plan_->ops[L];
+      uint32_t ref = 0;
+      for (int i = 0; i < op.nc; ++i) ref |= 1u << op.cond[i].v;
+      if (op.kind == kCompute || op.kind == kComputePathCount || op.kind == kComputeCount)
+        for (int i = 0; i < op.nb; ++i) ref |= 1u << op.bconn[i];
+      need_after |= ref;
+    }
+    std::memset(compute_bconn_, 0, sizeof(compute_bconn_));
+    for (int L = 0; L < n; ++L) {
+      const OpDesc& op = plan_->ops[L];
+      if ((op.kind == kCompute || op.kind == kComputePathCount) && !compute_bconn_[op.u])
+        for (int i = 0; i < op.nb; ++i) compute_bconn_[op.u] |= 1u << op.bconn[i];
+    }
+  }
+
+  uint64_t ReadU64(const uint64_t* dptr) {
+    SME_HIP(hipMemcpyAsync(ctx_->pinned(), dptr, sizeof(uint64_t), hipMemcpyDeviceToHost, st_));
+    SME_HIP(hipStreamSynchronize(st_));
+    stats_.d2h_syncs++;
+    return ctx_->pinned()[0];
+  }
+
+  uint64_t Budget(int L) const {
+    const uint64_t avail = std::min<uint64_t>(pool_->Available(), pool_->LargestFree());
+    const uint64_t per = static_cast<uint64_t>(0.9 * static_cast<double>(avail) /
+                                               static_cast<double>(plan_->nops - L));
+    return std::max<uint64_t>(per, 4096);
+  }
+
+  // Timing scope around one launch.
+  class Timed {
+   public:
+    Timed(Context* c, hipStream_t st, int slot, double bytes, uint64_t items)
+        : c_(c), st_(st), slot_(slot), bytes_(bytes), items_(items) {
+      if (c_->profiling) {
+        start_ = c_->TakeEvent();
+        SME_HIP(hipEventRecord(start_, st_));
+      }
+    }
+    ~Timed() {
+      if (!c_->profiling || !start_) return;
+      hipEvent_t stop = nullptr;
+      if (!c_->spare_events.empty()) {
+        stop = c_->spare_events.back();
+        c_->spare_events.pop_back();
+      } else if (hipEventCreate(&stop) != hipSuccess) {
+        c_->spare_events.push_back(start_);
+        return;
+      }
+      if (hipEventRecord(stop, st_) != hipSuccess) {
+        c_->spare_events.push_back(stop);
+        c_->spare_events.push_back(start_);
+        return;
+      }
+      c_->marks.push_back({start_, stop, slot_, bytes_, items_});
+    }
+
+   private:
+    Context* c_;
+    hipStream_t st_;
+    int slot_;
+    double bytes_;
+    uint64_t items_;
+    hipEvent_t start_ = nullptr;
+  };
+
+  void Scan(const uint64_t* in, uint64_t n, uint64_t* out) {
+    Buf<uint64_t> tmp(pool_, ScanTempWords(n));
+    Timed t(ctx_, st_, kSlotScan, 24.0 * n, n);
+    LaunchScanU64(in, n, out, tmp.get(), st_);
+  }
+  void Scan32(const uint32_t* in, uint64_t n, uint64_t* out) {
+    Buf<uint64_t> tmp(pool_, ScanTempWords(n));
+    Timed t(ctx_, st_, kSlotScan, 20.0 * n, n);
+    LaunchScanU32(in, n, out, tmp.get(), st_);
+  }
+
+  // Greedy split of paths [0, P) into batches whose cost
+  //   items * bi64 / 64 + paths * bp
+  // fits the level budget.  Returns right ends and the item offset at each end.
+  void Split(const uint64_t* offs, uint64_t P, uint64_t W, uint64_t bi64, uint64_t bp, int L,
+             std::vector<uint64_t>* ends, std::vector<uint64_t>* item_ends) {
+    const uint64_t budget = Budget(L);
+    const double total = static_cast<double>(W) * bi64 / 64.0 + static_cast<double>(P) * bp;
+    ends->clear();
+    item_ends->clear();
+    if (total <= static_cast<double>(budget) && W <= kMaxBatchItems && P <= kMaxBatchPaths) {
+      ends->push_back(P);
+      item_ends->push_back(W);
+      return;
+    }
+    const uint64_t want = static_cast<uint64_t>(total / budget) * 2 + 64;
+    const uint64_t max_out = std::min<uint64_t>(want, P);
+    // layout: [count, end_0, items_0, end_1, items_1, ...]
+    Buf<uint64_t> bounds(pool_, 2 * max_out + 2);
+    std::vector<uint64_t> host(2 * max_out);
+    uint64_t start = 0;
+    while (start < P) {
+      {
+        Timed t(ctx_, st_, kSlotMisc, 0, 0);
+        LaunchSplitBatches(offs, start, P, bi64, bp * 64, budget * 64, kMaxBatchItems, kMaxBatchPaths,
+                           bounds.get() + 1, max_out, bounds.get(), st_);
+      }
+      const uint64_t k = ReadU64(bounds.get());
+      if (k == 0) Fail(kErrInvalid, "batch split made no progress");
+      SME_HIP(hipMemcpyAsync(host.data(), bounds.get() + 1, 2 * k * sizeof(uint64_t),
+                             hipMemcpyDeviceToHost, st_));
+      SME_HIP(hipStreamSynchronize(st_));
+      stats_.d2h_syncs++;
+      for (uint64_t i = 0; i < k; ++i) {
+        ends->push_back(host[2 * i]);
+        item_ends->push_back(host[2 * i + 1]);
+      }
+      start = ends->back();
+    }
+  }
+
+  void Next(int L, const Frame& f) {
+    if (L + 1 < plan_->nops && f.P > 0) Level(L + 1, f);
+  }
+
+  // Per-item temporaries of a verify/compact pass, in 1/64 byte: mask (8),
+  // per-chunk count (4), per-chunk rank base (8), chunk partition (4), each
+  // per 64 items.
+  static constexpr uint64_t kChunkBytes64 = 8 + 4 + 8 + 4;
+  // Per-path arrays the next level allocates up front (size, pstart, offs,
+  // pivot id, scan scratch).
+  uint64_t NextPathBytes(int L) const { return L + 1 < plan_->nops ? 32 : 0; }
+
+  void Level0(uint32_t v_begin, uint32_t v_end) {
+    const OpDesc& op = plan_->ops[0];
+    const uint64_t total = v_end - v_begin;
+    // the next level keeps ~48 bytes of per-path state
+    const uint64_t per = std::max<uint64_t>(1, std::min<uint64_t>(Budget(0) / 48, kMaxBatchPaths));
+    for (uint64_t a = 0; a < total; a += per) {
+      const uint64_t b = std::min(total, a + per);
+      Frame f;
+      f.P = b - a;
+      f.cs.iota[op.u] = v_begin + static_cast<uint32_t>(a);
+      f.matmask = 1u << op.u;
+      host_level_[0] += f.P;
+      stats_.batches++;
+      Next(0, f);
+    }
+  }
+
+  void Level(int L, const Frame& f) {
+    const OpDesc& op = plan_->ops[L];
+    switch (op.kind) {
+      case kCompute:
+      case kComputePathCount:
+      case kComputeCount: ComputeLevel(L, f); break;
+      case kMaterialize: MaterializeLevel(L, f); break;
+      case kFilterCompute: FilterLevel(L, f); break;
+      case kCount: CountLevel(L, f); break;
+      default: Fail(kErrInvalid, "unknown op");
+    }
+  }
+
+  static void WindowConds(const OpDesc& op, uint8_t* nlt, uint8_t* lt, uint8_t* ngt, uint8_t* gt) {
+    *nlt = *ngt = 0;
+    for (int i = 0; i < op.nc; ++i) {
+      if (op.cond[i].op == kLess) lt[(*nlt)++] = op.cond[i].v;
+      if (op.cond[i].op == kLarger) gt[(*ngt)++] = op.cond[i].v;
+    }
+  }
+
+  // Non-equality conditions a candidate can actually violate.  x != M[w] is
+  // implied when x was drawn from N(M[w]) and the graph has no self loops.
+  uint8_t RealNonEqual(const OpDesc& op, uint32_t implied_mask, uint8_t* out) const {
+    uint8_t n = 0;
+    for (int i = 0; i < op.nc; ++i) {
+      if (op.cond[i].op != kNonEqual) continue;
+      const int w = op.cond[i].v;
+      if (!g_->self_loops && (implied_mask & (1u << w))) continue;
+      out[n++] = static_cast<uint8_t>(w);
+    }
+    return n;
+  }
+
+  ItemArgs BaseItems(const Frame& f, uint64_t l, uint64_t r, uint64_t items, const uint64_t* offs,
+                     const uint32_t* chunk_seg, uint64_t nchunks) const {
+    ItemArgs it{};
+    it.nitems = items;
+    it.nseg = r - l;
+    it.nchunks = nchunks;
+    it.offs = offs + l;
+    it.chunk_seg = chunk_seg;
+    it.rp = g_->rp;
+    it.cols = g_->cols;
+    it.cs = SliceCols(f.cs, l);
+    return it;
+  }
+
+  // --------------------------------------------------------------------------
+  void ComputeLevel(int L, const Frame& f) {
+    const OpDesc& op = plan_->ops[L];
+    const uint64_t P = f.P;
+    Buf<uint64_t> size(pool_, P), pstart(pool_, P), offs(pool_, P + 1);
+    Buf<uint8_t> piv(pool_, op.nb > 1 ? P : 0);
+    PrepComputeArgs pa{};
+    pa.npaths = P;
+    pa.rp = g_->rp;
+    pa.cols = g_->cols;
+    pa.cs = f.cs;
+    pa.nb = op.nb;
+    std::memcpy(pa.bconn, op.bconn, sizeof(pa.bconn));
+    WindowConds(op, &pa.nlt, pa.lt_v, &pa.ngt, pa.gt_v);
+    pa.size = size.get();
+    pa.pstart = pstart.get();
+    pa.piv = piv.get();
+    {
+      Timed t(ctx_, st_, kSlotPrep, P * (4.0 * (op.nb + pa.nlt + pa.ngt) + 16.0 * op.nb + 8.0 + 17.0), P);
+      LaunchPrepCompute(pa, st_);
+    }
+    Scan(size.get(), P, offs.get());
+    const uint64_t W = ReadU64(offs.get() + P);
+
+    uint32_t bmask = 0;
+    for (int i = 0; i < op.nb; ++i) bmask |= 1u << op.bconn[i];
+    uint8_t ne_v[kQ];
+    const uint8_t nne = RealNonEqual(op, bmask, ne_v);
+    const bool pair = op.kind == kComputeCount && op.ncun == 1;
+    const int u = op.u;
+
+    if (op.nb == 1 && nne == 0 && !pair) {
+      // the candidate set is a window of one adjacency list: no copy
+      if (op.kind == kComputeCount) {
+        host_level_[L] += W;
+        host_count_ += W;
+        return;
+      }
+      Frame f2 = f;
+      f2.cand[u] = CandRef{g_->cols, pstart.get(), size.get(), nullptr};
+      f2.candmask |= 1u << u;
+      host_level_[L] += W;
+      stats_.batches++;
+      Next(L, f2);
+      return;
+    }
+
+    // batch cost: per item (x1/64 byte) and per path, including the next
+    // level's per-path set-up for the paths that continue
+    const uint64_t next = NextPathBytes(L);
+    uint64_t bi64 = 4, bp = 0;
+    if (op.kind == kCompute) {
+      bi64 = 64 * 4 + kChunkBytes64;
+      bp = 16 + next;
+    } else if (op.kind == kComputePathCount) {
+      bi64 = kChunkBytes64;
+      bp = 8 + next;
+    }
+    std::vector<uint64_t> ends, item_ends;
+    Split(offs.get(), P, W, bi64, bp, L, &ends, &item_ends);
+    uint64_t l = 0, il = 0;
+    for (size_t b = 0; b < ends.size(); ++b) {
+      const uint64_t r = ends[b], ir = item_ends[b];
+      const uint64_t items = ir - il, nseg = r - l;
+      const uint64_t nchunks = (items + 63) / 64;
+      stats_.batches++;
+      Buf<uint32_t> chunk_seg(pool_, nchunks);
+      {
+        Timed t(ctx_, st_, kSlotPartition, 16.0 * nseg + 4.0 * nchunks, nseg);
+        LaunchChunkPartition(offs.get() + l, nseg, nchunks, chunk_seg.get(), st_);
+      }
+      ItemArgs it = BaseItems(f, l, r, items, offs.get(), chunk_seg.get(), nchunks);
+      it.src = g_->cols;
+      it.pstart = pstart.get() + l;
+      it.piv = op.nb > 1 ? piv.get() + l : nullptr;
+      it.nb = op.nb;
+      std::memcpy(it.bconn, op.bconn, sizeof(it.bconn));
+      it.nne = nne;
+      std::memcpy(it.ne_v, ne_v, sizeof(it.ne_v));
+      const double seg_bytes = 17.0 + 20.0 * (op.nb - 1) + 4.0 * nne;
+      if (op.kind == kComputeCount) {
+        if (pair) {
+          it.pair = SliceCand(f.cand[op.cun[0]], l);
+          it.pair_op = op.pair_op;
+        }
+        it.acc = acc_;
+        it.acc_level = acc_ + 1 + L;
+        Timed t(ctx_, st_, kSlotIntersect, 4.0 * op.nb * items + seg_bytes * nseg + (pair ? 20.0 * nseg : 0), items);
+        LaunchVerify(it, pair ? kModeSumPair : kModeSum, st_);
+        l = r;
+        il = ir;
+        continue;
+      }
+      Buf<unsigned long long> mask(pool_, nchunks);
+      Buf<uint32_t> ccount(pool_, nchunks);
+      it.mask = mask.get();
+      it.ccount = ccount.get();
+      {
+        Timed t(ctx_, st_, kSlotIntersect, 4.0 * op.nb * items + seg_bytes * nseg + 12.0 * nchunks, items);
+        LaunchVerify(it, kModeMask, st_);
+      }
+      Buf<uint64_t> cbase(pool_, nchunks + 1);
+      Scan32(ccount.get(), nchunks, cbase.get());
+      const uint64_t C = ReadU64(cbase.get() + nchunks);
+      RankRef rk{mask.get(), reinterpret_cast<const unsigned long long*>(cbase.get()), nchunks, items};
+      const bool write_vals = op.kind == kCompute;
+      Buf<uint32_t> vals(pool_, write_vals ? C : 0);
+      Buf<uint64_t> beg(pool_, write_vals ? nseg : 0), len(pool_, nseg);
+      CompactArgs ca{it, rk, vals.get(), beg.get(), len.get()};
+      {
+        Timed t(ctx_, st_, kSlotCompact, (write_vals ? 8.0 * C + 16.0 * nchunks : 0) + 32.0 * nseg, C);
+        LaunchCompact(ca, st_);
+      }
+      // the rank structures are not needed below this level
+      cbase.Reset();
+      ccount.Reset();
+      mask.Reset();
+      chunk_seg.Reset();
+      host_level_[L] += C;
+      Frame f2 = SliceFrame(f, l, r);
+      f2.cand[u] = CandRef{vals.get(), beg.get(), len.get(), nullptr};
+      f2.candmask |= 1u << u;
+      Next(L, f2);
+      l = r;
+      il = ir;
+    }
+  }
+
+  // --------------------------------------------------------------------------
+  void MaterializeLevel(int L, const Frame& f) {
+    const OpDesc& op = plan_->ops[L];
+    const int u = op.u;
+    const uint64_t P = f.P;
+    if (!(f.candmask & (1u << u))) Fail(kErrInvalid, "MATERIALIZE of a vertex with no candidates");
+    const CandRef cand = f.cand[u];
+    Buf<uint64_t> size(pool_, P), pstart(pool_, P), offs(pool_, P + 1);
+    PrepListArgs pa{};
+    pa.npaths = P;
+    pa.cs = f.cs;
+    pa.cand = cand;
+    WindowConds(op, &pa.nlt, pa.lt_v, &pa.ngt, pa.gt_v);
+    pa.size = size.get();
+    pa.pstart = pstart.get();
+    {
+      Timed t(ctx_, st_, kSlotPrep, P * (4.0 * (pa.nlt + pa.ngt) + 28.0), P);
+      LaunchPrepList(pa, st_);
+    }
+    Scan(size.get(), P, offs.get());
+    const uint64_t W = ReadU64(offs.get() + P);
+    uint8_t ne_v[kQ];
+    const uint8_t nne = RealNonEqual(op, compute_bconn_[u], ne_v);
+
+    // columns the deeper levels still read
+    uint8_t carry_v[kQ], ncarry = 0;
+    for (int i = 0; i < op.nmat; ++i) {
+      const int w = op.mat[i];
+      if (live_after_[L] & (1u << w)) carry_v[ncarry++] = static_cast<uint8_t>(w);
+    }
+    uint8_t idx_v[kQ], nidx = 0;
+    for (int w = 0; w < kQ; ++w)
+      if (w != u && (f.candmask & (1u << w))) idx_v[nidx++] = static_cast<uint8_t>(w);
+
+    // every surviving item becomes a path of the next level
+    const uint64_t bi64 = 64 * (4 * (1 + ncarry + nidx) + NextPathBytes(L)) + kChunkBytes64;
+    std::vector<uint64_t> ends, item_ends;
+    Split(offs.get(), P, W, bi64, 0, L, &ends, &item_ends);
+    uint64_t l = 0, il = 0;
+    for (size_t b = 0; b < ends.size(); ++b) {
+      const uint64_t r = ends[b], ir = item_ends[b];
+      const uint64_t items = ir - il, nseg = r - l;
+      const uint64_t nchunks = (items + 63) / 64;
+      stats_.batches++;
+      Buf<uint32_t> chunk_seg(pool_, nchunks);
+      {
+        Timed t(ctx_, st_, kSlotPartition, 16.0 * nseg + 4.0 * nchunks, nseg);
+        LaunchChunkPartition(offs.get() + l, nseg, nchunks, chunk_seg.get(), st_);
+      }
+      ItemArgs it = BaseItems(f, l, r, items, offs.get(), chunk_seg.get(), nchunks);
+      it.src = cand.base;
+      it.pstart = pstart.get() + l;
+      it.nb = 0;
+      it.nne = nne;
+      std::memcpy(it.ne_v, ne_v, sizeof(it.ne_v));
+      uint64_t C = items;
+      Buf<unsigned long long> mask(pool_, nne ? nchunks : 0);
+      Buf<uint32_t> ccount(pool_, nne ? nchunks : 0);
+      Buf<uint64_t> cbase(pool_, nne ? nchunks + 1 : 0);
+      RankRef rk{nullptr, nullptr, nchunks, items};
+      if (nne) {
+        it.mask = mask.get();
+        it.ccount = ccount.get();
+        {
+          Timed t(ctx_, st_, kSlotFilter, 4.0 * items + (16.0 + 4.0 * nne) * nseg + 12.0 * nchunks, items);
+          LaunchVerify(it, kModeMask, st_);
+        }
+        Scan32(ccount.get(), nchunks, cbase.get());
+        C = ReadU64(cbase.get() + nchunks);
+        rk.mask = mask.get();
+        rk.cbase = reinterpret_cast<const unsigned long long*>(cbase.get());
+      }
+      Buf<uint32_t> out_new(pool_, C);
+      std::vector<std::unique_ptr<Buf<uint32_t>>> carry, idx;
+      ExpandArgs ea{};
+      ea.it = it;
+      ea.rk = rk;
+      ea.out_new = out_new.get();
+      ea.ncarry = ncarry;
+      for (int i = 0; i < ncarry; ++i) {
+        carry.emplace_back(new Buf<uint32_t>(pool_, C));
+        ea.carry_v[i] = carry_v[i];
+        ea.carry_out[i] = carry.back()->get();
+      }
+      ea.nidx = nidx;
+      for (int i = 0; i < nidx; ++i) {
+        idx.emplace_back(new Buf<uint32_t>(pool_, C));
+        ea.idx_in[i] = SliceCand(f.cand[idx_v[i]], l);
+        ea.idx_out[i] = idx.back()->get();
+      }
+      {
+        Timed t(ctx_, st_, kSlotExpand,
+                4.0 * C * (2 + ncarry + nidx) + nseg * (16.0 + 4.0 * ncarry + 4.0 * nidx), C);
+        LaunchExpand(ea, st_);
+      }
+      cbase.Reset();
+      ccount.Reset();
+      mask.Reset();
+      chunk_seg.Reset();
+      host_level_[L] += C;
+      Frame f2;
+      f2.P = C;
+      f2.cs.inst[u] = out_new.get();
+      f2.matmask = 1u << u;
+      for (int i = 0; i < ncarry; ++i) {
+        f2.cs.inst[carry_v[i]] = carry[i]->get();
+        f2.matmask |= 1u << carry_v[i];
+      }
+      for (int i = 0; i < nidx; ++i) {
+        CandRef c = ea.idx_in[i];
+        c.idx = idx[i]->get();
+        f2.cand[idx_v[i]] = c;
+        f2.candmask |= 1u << idx_v[i];
+      }
+      Next(L, f2);
+      l = r;
+      il = ir;
+    }
+  }
+
+  // --------------------------------------------------------------------------
+  void FilterLevel(int L, const Frame& f) {
+    const OpDesc& op = plan_->ops[L];
+    const int u = op.u;
+    const uint64_t P = f.P;
+    if (!(f.candmask & (1u << u))) Fail(kErrInvalid, "FILTER_COMPUTE of a vertex with no candidates");
+    const CandRef cand = f.cand[u];
+    Buf<uint64_t> size(pool_, P), pstart(pool_, P), offs(pool_, P + 1);
+    PrepListArgs pa{};
+    pa.npaths = P;
+    pa.cs = f.cs;
+    pa.cand = cand;
+    WindowConds(op, &pa.nlt, pa.lt_v, &pa.ngt, pa.gt_v);
+    pa.size = size.get();
+    pa.pstart = pstart.get();
+    {
+      Timed t(ctx_, st_, kSlotPrep, P * (4.0 * (pa.nlt + pa.ngt) + 28.0), P);
+      LaunchPrepList(pa, st_);
+    }
+    Scan(size.get(), P, offs.get());
+    const uint64_t W = ReadU64(offs.get() + P);
+    uint8_t ne_v[kQ];
+    const uint8_t nne = RealNonEqual(op, compute_bconn_[u], ne_v);
+    if (nne == 0) {
+      Frame f2 = f;
+      f2.cand[u] = CandRef{cand.base, pstart.get(), size.get(), nullptr};
+      host_level_[L] += W;
+      stats_.batches++;
+      Next(L, f2);
+      return;
+    }
+    std::vector<uint64_t> ends, item_ends;
+    Split(offs.get(), P, W, 64 * 4 + kChunkBytes64, 16 + NextPathBytes(L), L, &ends, &item_ends);
+    uint64_t l = 0, il = 0;
+    for (size_t b = 0; b < ends.size(); ++b) {
+      const uint64_t r = ends[b], ir = item_ends[b];
+      const uint64_t items = ir - il, nseg = r - l;
+      const uint64_t nchunks = (items + 63) / 64;
+      stats_.batches++;
+      Buf<uint32_t> chunk_seg(pool_, nchunks);
+      {
+        Timed t(ctx_, st_, kSlotPartition, 16.0 * nseg + 4.0 * nchunks, nseg);
+        LaunchChunkPartition(offs.get() + l, nseg, nchunks, chunk_seg.get(), st_);
+      }
+      ItemArgs it = BaseItems(f, l, r, items, offs.get(), chunk_seg.get(), nchunks);
+      it.src = cand.base;
+      it.pstart = pstart.get() + l;
+      it.nb = 0;
+      it.nne = nne;
+      std::memcpy(it.ne_v, ne_v, sizeof(it.ne_v));
+      Buf<unsigned long long> mask(pool_, nchunks);
+      Buf<uint32_t> ccount(pool_, nchunks);
+      it.mask = mask.get();
+      it.ccount = ccount.get();
+      {
+        Timed t(ctx_, st_, kSlotFilter, 4.0 * items + (16.0 + 4.0 * nne) * nseg + 12.0 * nchunks, items);
+        LaunchVerify(it, kModeMask, st_);
+      }
+      Buf<uint64_t> cbase(pool_, nchunks + 1);
+      Scan32(ccount.get(), nchunks, cbase.get());
+      const uint64_t C = ReadU64(cbase.get() + nchunks);
+      RankRef rk{mask.get(), reinterpret_cast<const unsigned long long*>(cbase.get()), nchunks, items};
+      Buf<uint32_t> vals(pool_, C);
+      Buf<uint64_t> beg(pool_, nseg), len(pool_, nseg);
+      CompactArgs ca{it, rk, vals.get(), beg.get(), len.get()};
+      {
+        Timed t(ctx_, st_, kSlotCompact, 8.0 * C + 16.0 * nchunks + 32.0 * nseg, C);
+        LaunchCompact(ca, st_);
+      }
+      cbase.Reset();
+      ccount.Reset();
+      mask.Reset();
+      chunk_seg.Reset();
+      host_level_[L] += C;
+      Frame f2 = SliceFrame(f, l, r);
+      f2.cand[u] = CandRef{vals.get(), beg.get(), len.get(), nullptr};
+      Next(L, f2);
+      l = r;
+      il = ir;
+    }
+  }
+
+  // --------------------------------------------------------------------------
+  void CountLevel(int L, const Frame& f) {
+    const OpDesc& op = plan_->ops[L];
+    const uint64_t P = f.P;
+    if (plan_->direct_count || op.ncun == 1) {
+      CountPathArgs ca{};
+      ca.npaths = P;
+      ca.mode = plan_->direct_count ? 0 : 1;
+      ca.query_type = plan_->query_type;
+      ca.ncun = op.ncun;
+      std::memcpy(ca.cun, op.cun, sizeof(ca.cun));
+      for (int i = 0; i < op.ncun; ++i) ca.cand[op.cun[i]] = f.cand[op.cun[i]];
+      ca.acc = acc_;
+      ca.acc_level = acc_ + 1 + L;
+      Timed t(ctx_, st_, kSlotCount, P * 20.0 * op.ncun, P);
+      LaunchCountPaths(ca, st_);
+      return;
+    }
+    if (op.ncun != 2) Fail(kErrUnsupported, "COUNT needs one or two unmaterialized vertices");
+    const CandRef a = f.cand[op.cun[0]], b = f.cand[op.cun[1]];
+    Buf<uint64_t> size(pool_, P), pstart(pool_, P), offs(pool_, P + 1);
+    Buf<uint8_t> swapped(pool_, P);
+    PrepTwoArgs pa{P, a, b, size.get(), pstart.get(), swapped.get()};
+    {
+      Timed t(ctx_, st_, kSlotPrep, P * 57.0, P);
+      LaunchPrepTwo(pa, st_);
+    }
+    Scan(size.get(), P, offs.get());
+    const uint64_t W = ReadU64(offs.get() + P);
+    std::vector<uint64_t> ends, item_ends;
+    Split(offs.get(), P, W, 4, 0, L, &ends, &item_ends);
+    uint64_t l = 0, il = 0;
+    for (size_t bt = 0; bt < ends.size(); ++bt) {
+      const uint64_t r = ends[bt], ir = item_ends[bt];
+      const uint64_t items = ir - il, nseg = r - l;
+      const uint64_t nchunks = (items + 63) / 64;
+      stats_.batches++;
+      Buf<uint32_t> chunk_seg(pool_, nchunks);
+      {
+        Timed t(ctx_, st_, kSlotPartition, 16.0 * nseg + 4.0 * nchunks, nseg);
+        LaunchChunkPartition(offs.get() + l, nseg, nchunks, chunk_seg.get(), st_);
+      }
+      ItemArgs it = BaseItems(f, l, r, items, offs.get(), chunk_seg.get(), nchunks);
+      it.pstart = pstart.get() + l;
+      it.pair = SliceCand(a, l);
+      it.second = SliceCand(b, l);
+      it.swapped = swapped.get() + l;
+      it.pair_op = op.pair_op;
+      it.acc = acc_;
+      it.acc_level = acc_ + 1 + L;
+      Timed t(ctx_, st_, kSlotCount, 8.0 * items + 40.0 * nseg, items);
+      LaunchVerify(it, kModeSumTwo, st_);
+      l = r;
+      il = ir;
+    }
+  }
+
+  Context* ctx_;
+  const DevGraph* g_;
+  const EnginePlan* plan_;
+  hipStream_t st_;
+  DevicePool* pool_;
+  unsigned long long* acc_ = nullptr;
+  uint64_t host_level_[kMaxOps];
+  uint64_t levels_[kMaxOps];
+  uint64_t host_count_ = 0;
+  uint32_t live_after_[kMaxOps];
+  uint32_t compute_bconn_[kQ];
+  RunStats stats_;
+};
+
+}  // namespace
+
+Status RunPlan(Context* ctx, const DevGraph* g, const EnginePlan* plan, uint32_t v_begin,
+               uint32_t v_end, uint64_t* count, uint64_t* level_counts, RunStats* stats,
+               std::string* err) {
+  if (!ctx || !g || !plan || !count) {
+    *err = "null argument";
+    return kErrInvalid;
+  }
+  if (g->device != ctx->device()) {
+    *err = "graph lives on another device";
+    return kErrInvalid;
+  }
+  if (v_end > g->n) v_end = static_cast<uint32_t>(g->n);
+  if (v_begin > v_end) v_begin = v_end;
+  const auto t0 = std::chrono::steady_clock::now();
+  try {
+    SME_HIP(hipSetDevice(ctx->device()));
+    ctx->pool().ResetHighWater();
+    Runner r(ctx, g, plan);
+    RunStats st;
+    r.Run(v_begin, v_end, count, level_counts, &st);
+    if (ctx->profiling) ctx->ResolveMarks();
+    st.peak_pool_bytes = ctx->pool().HighWater();
+    st.elapsed_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (stats) *stats = st;
+  } catch (const Error& e) {
+    // drain the stream so pool memory handed back is no longer in use
+    (void)hipStreamSynchronize(ctx->stream());
+    for (auto& m : ctx->marks) {
+      ctx->spare_events.push_back(m.start);
+      ctx->spare_events.push_back(m.stop);
+    }
+    ctx->marks.clear();
+    *err = e.msg;
+    return e.status;
+  }
+  return kOk;
+}
+
+}  // namespace sme

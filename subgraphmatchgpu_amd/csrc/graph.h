@@ -1,0 +1,64 @@
+// graph.h -- host CSR graphs: the reference's BIN format, SNAP text input,
+// and the seeded synthetic generators used for the benchmark workloads.
+//
+// BIN layout (src/graph/GraphIO.h:88-136 read, :356-387 write):
+//   u64 sizeof(uintV)=4 | u64 sizeof(uintE)=8 | u64 n | u64 nnz |
+//   u64 row_ptrs[n+1] | u32 cols[nnz]
+// nnz counts directed entries (an undirected edge appears twice).
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "common.h"
+
+namespace sme {
+
+struct HostGraph {
+  uint64_t n = 0;
+  uint64_t nnz = 0;
+  std::vector<uintE> row_ptrs;  // n + 1
+  std::vector<uintV> cols;      // nnz
+};
+
+// Status-returning IO (no asserts, unlike GraphIO.h).
+Status ReadBin(const std::string& path, HostGraph* g, std::string* err);
+Status WriteBin(const std::string& path, const HostGraph& g, std::string* err);
+// SNAP edge list "u v" per line (GraphIO.h:232-311): ids shifted by the
+// minimum id, both directions when undirected, lists sorted.
+Status ReadSnap(const std::string& path, bool directed, HostGraph* g, std::string* err);
+
+// Structural checks used by the engine: row_ptrs monotone with
+// row_ptrs[n] == nnz, every id < n, every list strictly increasing.
+// `self_loops` reports whether any u appears in its own list.
+bool CheckCsr(uint64_t n, uint64_t nnz, const uintE* row_ptrs, const uintV* cols,
+              bool* self_loops, std::string* err);
+
+// Undirected simple graph from an edge list: self loops dropped, both
+// directions added, duplicates removed, lists sorted.  Ids stay 0..n-1 and
+// isolated vertices are kept.
+void BuildSimpleUndirected(uint64_t n, const std::vector<uint64_t>& packed_edges, HostGraph* g);
+
+// Seeded R-MAT: edge_factor * 2^scale draws, each of `scale` quadrant
+// choices with probabilities (a, b, c, 1-a-b-c); random stream SplitMix64 ->
+// xoshiro256**, one stream per fixed chunk of draws, so the result does not
+// depend on the thread count.
+void GenerateRmat(int scale, int edge_factor, double a, double b, double c, uint64_t seed,
+                  HostGraph* g);
+
+// Seeded Erdos-Renyi G(n, m): m distinct undirected edges drawn by rejection.
+void GenerateErdosRenyi(uint64_t n, uint64_t m, uint64_t seed, HostGraph* g);
+
+// Deterministic 64-bit PRNG pair used by the generators.
+struct SplitMix64 {
+  uint64_t s;
+  uint64_t Next();
+};
+struct Xoshiro256ss {
+  uint64_t s[4];
+  explicit Xoshiro256ss(uint64_t seed);
+  uint64_t Next();
+  double Uniform();  // [0, 1) with 53 random bits
+};
+
+}  // namespace sme

@@ -1,0 +1,186 @@
+// kernels.h -- argument blocks and host launchers of the gfx950 kernels.
+//
+// Every kernel takes its pointers and plan data as one by-value argument
+// block (kernarg segment), so no per-op pointer tables are copied to the
+// device (the reference uploads them before every op,
+// src/gpu/impl/light/LIGHTCommon.cuh:134-144).
+//
+// Work layout used by the item kernels: a level is a set of segments
+// (partial instances, "paths"); segment s owns items [offs[s], offs[s+1]).
+// One wave64 handles one chunk of 64 consecutive items; chunk_seg[c] is the
+// segment of the chunk's first item, so a lane finds its segment with a short
+// binary search over offs restricted to [chunk_seg[c], chunk_seg[c+1]].
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace sme {
+
+constexpr int kQ = 6;  // kMaxQueryVerticesNum
+
+// Materialized columns of a frame.  inst[w] == nullptr means the column is
+// the implicit sequence iota[w] + path (the first matched vertex).
+struct ColSet {
+  const uint32_t* inst[kQ];
+  uint32_t iota[kQ];
+};
+
+// Candidate set of one query vertex: segment k holds base[beg[k] ..
+// beg[k]+len[k]); path p uses segment idx[p] (idx == nullptr: segment p).
+struct CandRef {
+  const uint32_t* base;
+  const uint64_t* beg;
+  const uint64_t* len;
+  const uint32_t* idx;
+};
+
+// Per-path set-up of an intersection (COMPUTE / COMPUTE_PATH_COUNT /
+// COMPUTE_COUNT).  The pivot is the backward list of smallest degree; it is
+// cut to the value window implied by the < / > conditions.
+struct PrepComputeArgs {
+  uint64_t npaths;
+  const uint64_t* rp;
+  const uint32_t* cols;
+  ColSet cs;
+  uint8_t nb;
+  uint8_t bconn[kQ];
+  uint8_t nlt, ngt;
+  uint8_t lt_v[kQ];  // candidate < M[lt_v]
+  uint8_t gt_v[kQ];  // candidate > M[gt_v]
+  uint64_t* size;    // out: items of the path (restricted pivot length)
+  uint64_t* pstart;  // out: first item's index into cols
+  uint8_t* piv;      // out: which backward list is the pivot
+};
+
+// Per-path set-up of a scan over an existing candidate set (MATERIALIZE,
+// FILTER_COMPUTE): the segment is cut to the < / > window.
+struct PrepListArgs {
+  uint64_t npaths;
+  ColSet cs;
+  CandRef cand;
+  uint8_t nlt, ngt;
+  uint8_t lt_v[kQ];
+  uint8_t gt_v[kQ];
+  uint64_t* size;
+  uint64_t* pstart;
+};
+
+// Item kernels.  `offs` / per-segment arrays are already offset to the
+// batch's first segment; items are numbered from offs[0].
+struct ItemArgs {
+  uint64_t nitems;
+  uint64_t nseg;
+  uint64_t nchunks;
+  const uint64_t* offs;       // nseg + 1
+  const uint32_t* chunk_seg;  // nchunks
+  // value of an item: src[pstart[s] + rank]
+  const uint32_t* src;
+  const uint64_t* pstart;
+  // membership tests (intersection): every backward list but the pivot
+  const uint64_t* rp;
+  const uint32_t* cols;
+  const uint8_t* piv;
+  uint8_t nb;
+  uint8_t bconn[kQ];
+  // non-equality conditions that the window cannot express
+  uint8_t nne;
+  uint8_t ne_v[kQ];
+  ColSet cs;
+  // COMPUTE_COUNT with one other unmaterialized vertex
+  CandRef pair;
+  uint8_t pair_op;
+  // COUNT over two unmaterialized vertices: `pair` is the first vertex's
+  // set, `second` the other; swapped[s] == 1 when the second is smaller.
+  CandRef second;
+  const uint8_t* swapped;
+  // outputs
+  unsigned long long* mask;       // per chunk validity bits
+  uint32_t* ccount;               // per chunk popcount
+  unsigned long long* acc;        // total count
+  unsigned long long* acc_level;  // this level's contribution
+};
+
+enum VerifyMode : int { kModeMask = 0, kModeSum = 1, kModeSumPair = 2, kModeSumTwo = 3 };
+
+// Rank structures built from a verify pass.
+struct RankRef {
+  const unsigned long long* mask;   // nullptr: every item survives
+  const unsigned long long* cbase;  // exclusive scan of ccount (nchunks + 1)
+  uint64_t nchunks;
+  uint64_t nitems;
+};
+
+// MATERIALIZE output columns.
+struct ExpandArgs {
+  ItemArgs it;
+  RankRef rk;
+  uint32_t* out_new;              // column of the materialized vertex
+  uint8_t ncarry;
+  uint8_t carry_v[kQ];            // materialized columns copied to children
+  uint32_t* carry_out[kQ];
+  uint8_t nidx;
+  CandRef idx_in[kQ];             // candidate indices copied to children
+  uint32_t* idx_out[kQ];
+};
+
+// COMPUTE / FILTER_COMPUTE output: compacted values + per-segment window.
+struct CompactArgs {
+  ItemArgs it;
+  RankRef rk;
+  uint32_t* out_vals;
+  uint64_t* out_beg;
+  uint64_t* out_len;
+};
+
+// Per-path closed-form / single-set counts (COUNT).
+struct CountPathArgs {
+  uint64_t npaths;
+  uint8_t mode;  // 0: direct closed form, 1: size of one set
+  int query_type;
+  uint8_t ncun;
+  uint8_t cun[kQ];
+  CandRef cand[kQ];  // indexed by query vertex
+  unsigned long long* acc;
+  unsigned long long* acc_level;
+};
+
+// Per-path set-up of the two-set COUNT.
+struct PrepTwoArgs {
+  uint64_t npaths;
+  CandRef a, b;
+  uint64_t* size;
+  uint64_t* pstart;
+  uint8_t* swapped;
+};
+
+// ---------------------------------------------------------------------------
+// Launchers (all on `stream`).
+void LaunchPrepCompute(const PrepComputeArgs& a, hipStream_t stream);
+void LaunchPrepList(const PrepListArgs& a, hipStream_t stream);
+void LaunchPrepTwo(const PrepTwoArgs& a, hipStream_t stream);
+void LaunchCountPaths(const CountPathArgs& a, hipStream_t stream);
+void LaunchChunkPartition(const uint64_t* offs, uint64_t nseg, uint64_t nchunks,
+                          uint32_t* chunk_seg, hipStream_t stream);
+void LaunchVerify(const ItemArgs& a, int mode, hipStream_t stream);
+void LaunchCompact(const CompactArgs& a, hipStream_t stream);
+void LaunchExpand(const ExpandArgs& a, hipStream_t stream);
+void LaunchIota32(uint32_t* out, uint64_t n, uint32_t base, hipStream_t stream);
+void LaunchFill64(uint64_t* out, uint64_t n, uint64_t value, hipStream_t stream);
+// Greedy batch split over paths [start, npaths): writes up to max_out pairs
+// (right end, offs[right end]) to `bounds` and the pair count to counters[0].
+void LaunchSplitBatches(const uint64_t* offs, uint64_t start, uint64_t npaths,
+                        uint64_t bytes_per_item, uint64_t bytes_per_path, uint64_t budget,
+                        uint64_t max_items, uint64_t max_paths, uint64_t* bounds,
+                        uint64_t max_out, uint64_t* counters, hipStream_t stream);
+
+// Exclusive scan: out[i] = sum(in[0..i)), out[n] = total.  `tmp` needs
+// ScanTempWords(n) u64 words.  out may alias in.
+size_t ScanTempWords(uint64_t n);
+void LaunchScanU64(const uint64_t* in, uint64_t n, uint64_t* out, uint64_t* tmp,
+                   hipStream_t stream);
+void LaunchScanU32(const uint32_t* in, uint64_t n, uint64_t* out, uint64_t* tmp,
+                   hipStream_t stream);
+
+}  // namespace sme

@@ -1,0 +1,91 @@
+"""The engine's own sources (kernels + host orchestration) executed on the CPU
+by the fiber-based stand-in runtime in tests/emu, under AddressSanitizer and
+UBSan, compared with the oracle.  This catches index arithmetic errors before
+any kernel reaches the GPU."""
+import os
+import subprocess
+
+import pytest
+
+import oracle
+import subgraphmatchgpu_amd as sme
+from conftest import ROOT, clique
+
+EMU_DIR = os.path.join(ROOT, "tests", "emu")
+EMU = os.path.join(EMU_DIR, "emu_driver")
+
+
+@pytest.fixture(scope="module")
+def emu():
+    subprocess.run(["make", "-C", EMU_DIR, "-s"], check=True)
+    return EMU
+
+
+def run_emu(emu, path, algo, q, cl, vb, ve, budget, ordering=1):
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0")
+    r = subprocess.run([emu, path, str(algo), str(q), str(cl), str(ordering), str(vb), str(ve),
+                        str(budget)], capture_output=True, text=True, env=env, timeout=600)
+    assert r.returncode == 0, r.stderr[-3000:]
+    parts = r.stdout.split()
+    cnt = int(parts[0].split("=")[1])
+    lv = [int(x) for x in parts[1].split("=")[1].split(",")]
+    return cnt, lv, int(parts[2].split("=")[1])
+
+
+@pytest.fixture(scope="module")
+def graphs(tmp_path_factory):
+    d = tmp_path_factory.mktemp("emu")
+    out = {}
+    for name, g in {"rmat6": sme.gen_rmat(6, 8, seed=3), "rmat7": sme.gen_rmat(7, 6, seed=3),
+                    "er60": sme.gen_er(60, 240, seed=2), "k7": clique(7)}.items():
+        p = str(d / (name + ".bin"))
+        sme.write_bin(p, g)
+        out[name] = (g, p)
+    return out
+
+
+@pytest.mark.parametrize("q", list(range(14)))
+def test_emu_light_all_compress_levels(emu, graphs, q):
+    for name in ("rmat6", "er60"):
+        g, p = graphs[name]
+        for cl in range(4):
+            want = oracle.light_count(g.row_ptrs, g.cols, q, cl)
+            got = run_emu(emu, p, 6, q, cl, 0, g.n, 1 << 30)
+            assert got[:2] == (want[0], want[1]), (name, cl)
+
+
+@pytest.mark.parametrize("q", [0, 1, 2, 3, 4, 7, 9, 13])
+def test_emu_gpsm(emu, graphs, q):
+    g, p = graphs["rmat7"]
+    want, wl = oracle.gpsm_count(g.row_ptrs, g.cols, q)
+    cnt, lv, _ = run_emu(emu, p, 4, q, 0, 0, g.n, 1 << 30)
+    assert cnt == want
+    # the engine's levels alternate C/M; the M levels are GPSM's level sizes
+    assert [lv[0]] + lv[2::2] == wl[:len(lv[2::2]) + 1]
+
+
+def test_emu_tiny_budget_invariance(emu, graphs):
+    g, p = graphs["rmat7"]
+    for q in (1, 3, 4, 7):
+        want = oracle.light_count(g.row_ptrs, g.cols, q, 3)
+        seen = set()
+        for budget in (40_000, 1 << 30):
+            cnt, lv, batches = run_emu(emu, p, 6, q, 3, 0, g.n, budget)
+            assert (cnt, lv) == (want[0], want[1]), (q, budget)
+            seen.add(batches)
+        assert len(seen) > 1, "budgets did not change the batching"
+
+
+def test_emu_ranges_and_empty(emu, graphs):
+    g, p = graphs["rmat7"]
+    for q in (0, 4):
+        for a, b in ((0, 0), (5, 5), (0, 17), (17, 90), (90, g.n)):
+            want = oracle.light_count(g.row_ptrs, g.cols, q, 3, v_begin=a, v_end=b)
+            assert run_emu(emu, p, 6, q, 3, a, b, 1 << 30)[:2] == (want[0], want[1])
+
+
+def test_emu_clique_without_ordering(emu, graphs):
+    g, p = graphs["k7"]
+    for q in (0, 3, 7):
+        want = oracle.light_count(g.row_ptrs, g.cols, q, 3, enable_ordering=False)
+        assert run_emu(emu, p, 6, q, 3, 0, g.n, 1 << 30, ordering=0)[:2] == (want[0], want[1])

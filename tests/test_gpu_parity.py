@@ -1,0 +1,158 @@
+"""HIP engine (libsme.so on the MI355X, through the C ABI) against the oracle
+and the reference's golden counts.  Bit-exact: counts and per-level outputs
+are integers and must be identical."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+import subgraphmatchgpu_amd as sme
+from conftest import clique
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def dev():
+    d = sme.Device(0, budget_bytes=16 << 30)
+    yield d
+    d.close()
+
+
+@pytest.fixture(scope="module")
+def small(dev, fixture_graphs):
+    return {name: (g, dev.upload(g)) for name, g in fixture_graphs.items()
+            if name in ("rmat8", "rmat10", "er300")}
+
+
+@pytest.mark.parametrize("q", list(range(14)))
+def test_light_all_compress_levels(dev, small, q):
+    for name in ("rmat8", "er300"):
+        g, dg = small[name]
+        for cl in range(4):
+            say(f"case light q={q} cl={cl} graph={name}")
+            want = oracle.light_count(g.row_ptrs, g.cols, q, cl)
+            res = dev.count(dg, sme.Plan(q, sme.GPU_LIGHT, cl))
+            assert (res.count, res.levels) == want, (name, cl)
+
+
+@pytest.mark.parametrize("q", list(range(14)))
+def test_gpsm(dev, small, q):
+    g, dg = small["rmat8"]
+    say(f"case gpsm q={q}")
+    want, wl = oracle.gpsm_count(g.row_ptrs, g.cols, q)
+    res = dev.count(dg, sme.Plan(q, sme.GPU_GPSM))
+    assert res.count == want
+    assert [res.levels[0]] + res.levels[2::2] == wl[:len(res.levels[2::2]) + 1]
+
+
+def test_golden_reference_counts(dev, ref_counts, fixture_graphs):
+    """Counts printed by the reference CPU binary (tests/golden/ref_counts.json)."""
+    for name, res in ref_counts["counts"].items():
+        g = fixture_graphs[name]
+        dg = dev.upload(g)
+        for q, want in res["wcoj"].items():
+            say(f"case golden graph={name} q={q}")
+            got = dev.count(dg, sme.Plan(int(q), sme.GPU_LIGHT, sme.CL_SPECIAL))
+            assert got.count == want, (name, q)
+            if int(q) in (0, 3, 7):
+                assert dev.count(dg, sme.Plan(int(q), sme.GPU_GPSM)).count == want, (name, q)
+        dg.release()
+
+
+def test_config_small_triangle(dev, ref_counts, fixture_graphs):
+    """RMAT scale 14, edge factor 16, triangle: exact count and per-level
+    partial instances {n, nnz/2, nnz/2, T}."""
+    g = fixture_graphs["rmat14"]
+    dg = dev.upload(g)
+    res = dev.count(dg, sme.Plan(sme.Q0))
+    T = ref_counts["counts"]["rmat14"]["wcoj"]["0"]
+    assert res.count == T
+    assert res.levels == [g.n, g.nnz // 2, g.nnz // 2, T]
+    dg.release()
+
+
+def test_cliques_closed_form(dev):
+    import math
+
+    g = clique(12)
+    dg = dev.upload(g)
+    C = math.comb
+    forms = {0: C(12, 3), 1: 3 * C(12, 4), 2: 6 * C(12, 4), 3: C(12, 4), 4: 60 * C(12, 5),
+             7: C(12, 5), 11: C(12, 6)}
+    for q, want in forms.items():
+        say(f"case clique q={q}")
+        for cl in range(4):
+            assert dev.count(dg, sme.Plan(q, sme.GPU_LIGHT, cl)).count == want
+    dg.release()
+
+
+def test_batching_invariance(fixture_graphs):
+    """A tight device budget forces many batches; counts and per-level outputs
+    must not change."""
+    g = fixture_graphs["rmat12"]
+    big = sme.Device(0, budget_bytes=8 << 30)
+    tiny = sme.Device(0, budget_bytes=3 << 20)
+    gb, gt = big.upload(g), tiny.upload(g)
+    for q in (0, 2, 3, 4, 7):
+        say(f"case batching q={q}")
+        a = big.count(gb, sme.Plan(q))
+        b = tiny.count(gt, sme.Plan(q))
+        assert (a.count, a.levels) == (b.count, b.levels)
+        assert b.batches > a.batches
+    big.close()
+    tiny.close()
+
+
+def test_start_ranges(dev, small):
+    g, dg = small["rmat10"]
+    cuts = [0, 1, 7, 300, 301, 700, g.n]
+    for q in (0, 1, 2, 4, 7):
+        say(f"case ranges q={q}")
+        whole = dev.count(dg, sme.Plan(q)).count
+        parts = 0
+        for a, b in zip(cuts[:-1], cuts[1:]):
+            r = dev.count(dg, sme.Plan(q), a, b)
+            want = oracle.light_count(g.row_ptrs, g.cols, q, 3, v_begin=a, v_end=b)
+            assert (r.count, r.levels) == want
+            parts += r.count
+        assert parts == whole
+    assert dev.count(dg, sme.Plan(0), 5, 5).count == 0
+    assert dev.count(dg, sme.Plan(0), 10, 3).count == 0
+
+
+def test_edge_graphs(dev):
+    empty = sme.from_adjacency([[] for _ in range(10)])
+    dg = dev.upload(empty)
+    for q in range(14):
+        assert dev.count(dg, sme.Plan(q)).count == 0
+    star = sme.from_adjacency([list(range(1, 50))] + [[0] for _ in range(49)])
+    ds = dev.upload(star)
+    assert dev.count(ds, sme.Plan(0)).count == 0
+    assert dev.count(ds, sme.Plan(1)).count == 0
+
+
+def test_rejects_bad_csr(dev):
+    bad = sme.Graph(np.array([0, 2, 3, 4], dtype=np.uint64), np.array([2, 1, 0, 0], dtype=np.uint32))
+    with pytest.raises(sme.SmeError) as e:
+        dev.upload(bad)
+    assert e.value.status == 1
+
+
+def test_self_loops_handled(dev):
+    # a self loop must not satisfy non-equality conditions
+    adj = [[0, 1, 2], [0, 1, 2], [0, 1, 2]]
+    g = sme.from_adjacency(adj)
+    dg = dev.upload(g)
+    for q in (0, 1, 2, 3, 4):
+        want = oracle.wcoj_count(g.row_ptrs, g.cols, q)
+        lw = oracle.light_count(g.row_ptrs, g.cols, q, 3)
+        got = dev.count(dg, sme.Plan(q))
+        assert (got.count, got.levels) == lw
+        assert got.count == want

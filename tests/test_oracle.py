@@ -1,0 +1,178 @@
+"""Pins the CPU oracle (oracle/oracle.c) before anything is checked against it.
+
+Each check compares the oracle with something it was not derived from:
+the reference's own plan classes and binary (golden fixtures, and the live
+binary when oracle/_ref is built), closed forms on cliques, and a brute-force
+monomorphism count on tiny random graphs.
+"""
+import itertools
+import math
+import os
+import random
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import GOLDEN, clique
+
+# Appendix of closed forms: counts of each pattern in K_n with symmetry
+# breaking (each unordered instance once).
+def kn_closed_form(q, n):
+    C = math.comb
+    return {0: C(n, 3), 1: 3 * C(n, 4), 2: 6 * C(n, 4), 3: C(n, 4), 4: 60 * C(n, 5), 7: C(n, 5),
+            11: C(n, 6)}[q]
+
+
+def test_plans_match_reference_plan_classes():
+    with open(os.path.join(GOLDEN, "light_plans.txt")) as f:
+        golden = f.read()
+    mine = "".join(oracle.plan_text(q, cl, o) for o in (True, False) for q in range(14) for cl in range(4))
+    assert mine == golden
+
+
+def test_appendix_plans():
+    # LIGHT execute sequences of the six benchmark queries (default compress level)
+    ops = {q: oracle.plan_text(q, 3).split("\n")[1] for q in (0, 1, 2, 3, 4, 7)}
+    assert ops[0] == "ops: M0 C1 M1 CC2"
+    assert ops[1] == "ops: M0 C1 M1 C2 M2 CC3"
+    assert ops[2] == "ops: M0 C2 M2 CPC1 CPC3 COUNT4"
+    assert ops[3] == "ops: M0 C1 M1 C2 M2 CC3"
+    assert ops[4] == "ops: M1 C2 C3 M2 C0 M3 FC0 CC4"
+    assert ops[7] == "ops: M0 C1 M1 C2 M2 C3 M3 CC4"
+
+
+@pytest.mark.parametrize("n", [5, 6, 8])
+@pytest.mark.parametrize("q", [0, 1, 2, 3, 4, 7, 11])
+def test_cliques_closed_form(q, n):
+    if q == 11 and n < 6:
+        pytest.skip("needs 6 vertices")
+    g = clique(n)
+    want = kn_closed_form(q, n)
+    assert oracle.wcoj_count(g.row_ptrs, g.cols, q) == want
+    for cl in range(4):
+        assert oracle.light_count(g.row_ptrs, g.cols, q, cl)[0] == want
+    assert oracle.gpsm_count(g.row_ptrs, g.cols, q)[0] == want
+
+
+# |Aut(P)| of each pattern (number of automorphisms)
+AUT = {0: 6, 1: 8, 2: 4, 3: 24, 4: 2, 7: 120}
+PATTERN_EDGES = {
+    0: [(0, 1), (0, 2), (1, 2)],
+    1: [(0, 1), (1, 2), (2, 3), (0, 3)],
+    2: [(0, 1), (0, 2), (0, 3), (1, 2), (2, 3)],
+    3: [(a, b) for a, b in itertools.combinations(range(4), 2)],
+    4: [(0, 1), (0, 2), (1, 2), (1, 3), (2, 4), (3, 4)],
+    7: [(a, b) for a, b in itertools.combinations(range(5), 2)],
+}
+
+
+def brute_monomorphisms(adj, q):
+    """Injective maps of the pattern into the graph preserving pattern edges."""
+    k = 1 + max(max(e) for e in PATTERN_EDGES[q])
+    n = len(adj)
+    sets = [set(a) for a in adj]
+    cnt = 0
+    for perm in itertools.permutations(range(n), k):
+        if all(perm[b] in sets[perm[a]] for a, b in PATTERN_EDGES[q]):
+            cnt += 1
+    return cnt
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_brute_force_small_random(seed):
+    import subgraphmatchgpu_amd as sme
+
+    rng = random.Random(seed)
+    n = 9
+    adj = [set() for _ in range(n)]
+    for a, b in itertools.combinations(range(n), 2):
+        if rng.random() < 0.55:
+            adj[a].add(b)
+            adj[b].add(a)
+    g = sme.from_adjacency([sorted(s) for s in adj])
+    for q, aut in AUT.items():
+        want = brute_monomorphisms([sorted(s) for s in adj], q)
+        assert want % aut == 0
+        want //= aut
+        assert oracle.wcoj_count(g.row_ptrs, g.cols, q) == want, q
+        assert oracle.light_count(g.row_ptrs, g.cols, q, 3)[0] == want, q
+
+
+def _fixture_cases(ref_counts, max_name=None):
+    cases = []
+    for name, res in ref_counts["counts"].items():
+        for q, c in res["wcoj"].items():
+            cases.append((name, int(q), c))
+    return cases
+
+
+SLOW = {("rmat10", 5), ("rmat10", 13), ("rmat10", 4), ("rmat14", 7), ("rmat14", 3), ("rmat14", 2),
+        ("rmat12", 1), ("rmat12", 6), ("rmat12", 11), ("rmat12", 7)}
+
+
+def test_oracle_vs_reference_binary_fixture(ref_counts, fixture_graphs):
+    """WCOJ and LIGHT restatements against the reference binary's counts."""
+    for name, q, want in _fixture_cases(ref_counts):
+        if (name, q) in SLOW:
+            continue
+        g = fixture_graphs[name]
+        assert oracle.wcoj_count(g.row_ptrs, g.cols, q) == want, (name, q)
+        assert oracle.light_count(g.row_ptrs, g.cols, q, 3)[0] == want, (name, q)
+
+
+def test_reference_algorithms_agree(ref_counts):
+    for name, res in ref_counts["counts"].items():
+        for algo in ("bfs", "rbfs", "vf2"):
+            for q, c in res.get(algo, {}).items():
+                assert c == res["wcoj"][q], (name, algo, q)
+
+
+def test_triangle_levels_closed_form(fixture_graphs):
+    """Q0 LIGHT levels: M0 = n, C1 = M1 = nnz/2 (each undirected edge once,
+    v1 > v0), CC2 = triangles."""
+    for name in ("rmat8", "rmat12", "rmat14", "er300"):
+        g = fixture_graphs[name]
+        cnt, lv = oracle.light_count(g.row_ptrs, g.cols, 0, 3)
+        assert lv == [g.n, g.nnz // 2, g.nnz // 2, cnt]
+
+
+def test_level_outputs_consistent(fixture_graphs):
+    """Every compress level gives the same total; GPSM's last level is the
+    count and its first level is the start range."""
+    g = fixture_graphs["rmat8"]
+    for q in range(14):
+        totals = {oracle.light_count(g.row_ptrs, g.cols, q, cl)[0] for cl in range(4)}
+        assert len(totals) == 1, q
+        c, lv = oracle.gpsm_count(g.row_ptrs, g.cols, q)
+        assert c in totals and lv[0] == g.n
+
+
+def test_start_range_additivity(fixture_graphs):
+    g = fixture_graphs["rmat10"]
+    for q in (0, 2, 3, 4, 7):
+        whole = oracle.light_count(g.row_ptrs, g.cols, q, 3)[0]
+        parts = sum(oracle.light_count(g.row_ptrs, g.cols, q, 3, v_begin=a, v_end=b)[0]
+                    for a, b in ((0, 100), (100, 517), (517, g.n)))
+        assert parts == whole
+
+
+@pytest.mark.skipif(not oracle.ref_available(), reason="reference CPU binary not built")
+def test_oracle_vs_live_reference(tmp_path):
+    import subgraphmatchgpu_amd as sme
+
+    g = sme.gen_rmat(9, 6, seed=99)
+    p = str(tmp_path / "g.bin")
+    sme.write_bin(p, g)
+    for q in (0, 1, 2, 3, 4, 7, 8, 9):
+        want, _ = oracle.ref_count(p, q, algo=0, threads=4)
+        assert oracle.wcoj_count(g.row_ptrs, g.cols, q) == want, q
+        assert oracle.light_count(g.row_ptrs, g.cols, q, 3)[0] == want, q
+
+
+def test_planted_mistake_is_caught():
+    """A plausible oracle mistake (dropping the symmetry-breaking orders)
+    must change the count the pins compare against."""
+    g = clique(6)
+    assert oracle.wcoj_count(g.row_ptrs, g.cols, 0, enable_ordering=False) == 6 * kn_closed_form(0, 6)
+    assert oracle.wcoj_count(g.row_ptrs, g.cols, 0, enable_ordering=False) != kn_closed_form(0, 6)
