@@ -33,38 +33,34 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(g, seconds_target=15.0):
-    """Oracle port (CPU_WCOJ restatement, OpenMP over all host threads) on
-    evenly spaced start-vertex windows of the same graph and queries."""
+def cpu_baseline(g, seconds_target=15.0, chunk=64, seed=12345):
+    """Oracle port of the reference's CPU_WCOJ (OpenMP over all host
+    threads) on a seeded uniform random sample of start vertices of the same
+    graph, both queries, stopped once about `seconds_target` seconds of CPU
+    work are done."""
+    import numpy as np
+
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
 
     threads = os.cpu_count() or 1
-    n = g.n
-    windows = 256
-    width = 8
-    total_emb, t_used, covered = 0, 0.0, 0
-    # widen the sample until it costs about seconds_target
-    while True:
-        emb = 0
+    order = np.random.default_rng(seed).permutation(g.n).astype(np.uint32)
+    emb, used, done = 0, 0.0, 0
+    while done < g.n and used < seconds_target:
+        starts = np.sort(order[done:done + chunk])
         t0 = time.perf_counter()
-        for w in range(windows):
-            a = (w * n) // windows
-            b = min(n, a + width)
-            for q in QUERIES:
-                emb += oracle.wcoj_count(g.row_ptrs, g.cols, q, True, a, b, threads)
-        dt = time.perf_counter() - t0
-        total_emb, t_used, covered = emb, dt, windows * width
-        if dt > seconds_target / 4 or width >= n // windows:
-            break
-        width = min(n // windows, width * 4)
+        for q in QUERIES:
+            emb += oracle.wcoj_count_list(g.row_ptrs, g.cols, q, starts, True, threads)
+        used += time.perf_counter() - t0
+        done += starts.size
+        log(f"cpu baseline: {done} start vertices, {emb} embeddings, {used:.1f}s")
     return {
-        "value": total_emb / t_used if t_used > 0 else 0.0,
+        "value": emb / used if used > 0 else 0.0,
         "unit": "embeddings/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"oracle CPU_WCOJ port, Q3+Q2 over {windows} evenly spaced windows of {covered // windows} "
-                  f"start vertices ({covered} of {n}); {total_emb} embeddings in {t_used:.2f} s",
+        "sample": f"oracle CPU_WCOJ port (restates src/cpu/CPUWCOJ.h), Q3+Q2 from {done} seeded random "
+                  f"start vertices of {g.n}; {emb} embeddings in {used:.2f} s",
     }
 
 
@@ -91,8 +87,12 @@ def main():
 
     def step():
         counts = []
-        for p in plans:
-            counts.append(dev.count(dg, p).count)
+        for q, p in zip(QUERIES, plans):
+            t = time.perf_counter()
+            r = dev.count(dg, p)
+            counts.append(r.count)
+            log(f"  Q{q}: {r.count} embeddings in {1e3 * (time.perf_counter() - t):.1f} ms, "
+                f"{r.batches} batches, {r.d2h_syncs} syncs, peak pool {r.peak_pool_bytes / 2**30:.2f} GiB")
         return counts
 
     for i in range(args.warmup):

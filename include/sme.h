@@ -102,6 +102,12 @@ int sme_plan_create(int algo, int query_type, int compress_level, int enable_ord
 int sme_plan_describe(const sme_plan* p, char* buf, size_t cap);
 /* Number of exec levels (length of the level_counts array of sme_count). */
 int sme_plan_levels(const sme_plan* p);
+/* Intersection reuse across levels (on by default for LIGHT and
+ * GPSM_REUSE, off for GPSM): a level whose backward lists include the
+ * previous vertex's lists extends that vertex's candidate segment instead of
+ * re-intersecting adjacency lists, where the conditions prove the result is
+ * the same set.  Counterpart of the reuse plans, query/ReuseTraversalPlan.h:9-77. */
+int sme_plan_set_reuse(sme_plan* p, int on);
 void sme_plan_destroy(sme_plan* p);
 
 /* ---- execution -------------------------------------------------------- */

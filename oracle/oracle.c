@@ -431,12 +431,33 @@ static uint64_t wcoj_dfs(wcoj_ctx* c, int lvl, uint32_t* path) {
   return ans;
 }
 
+static int wcoj_run(uint64_t n, const uint64_t* row_ptrs, const uint32_t* cols, int q,
+                    int enable_ordering, uint32_t v_begin, uint32_t v_end, const uint32_t* starts,
+                    int nthreads, uint64_t* out_count);
+
 int or_wcoj_count(uint64_t n, const uint64_t* row_ptrs, const uint32_t* cols, int q,
                   int enable_ordering, uint32_t v_begin, uint32_t v_end, int nthreads,
                   uint64_t* out_count) {
+  if (v_end > n) v_end = (uint32_t)n;
+  if (v_begin > v_end) v_begin = v_end;
+  return wcoj_run(n, row_ptrs, cols, q, enable_ordering, v_begin, v_end, NULL, nthreads, out_count);
+}
+
+int or_wcoj_count_list(uint64_t n, const uint64_t* row_ptrs, const uint32_t* cols, int q,
+                       int enable_ordering, const uint32_t* starts, uint64_t nstarts,
+                       int nthreads, uint64_t* out_count) {
+  for (uint64_t i = 0; i < nstarts; ++i)
+    if (starts[i] >= n) return -1;
+  return wcoj_run(n, row_ptrs, cols, q, enable_ordering, 0, (uint32_t)nstarts, starts, nthreads,
+                  out_count);
+}
+
+/* starts == NULL: start vertices v_begin..v_end-1; else starts[v_begin..v_end). */
+static int wcoj_run(uint64_t n, const uint64_t* row_ptrs, const uint32_t* cols, int q,
+                    int enable_ordering, uint32_t v_begin, uint32_t v_end, const uint32_t* starts,
+                    int nthreads, uint64_t* out_count) {
   or_query qq;
   if (or_query_init(q, enable_ordering, &qq)) return -1;
-  if (v_end > n) v_end = (uint32_t)n;
   graph_t g = {n, row_ptrs, cols};
   uint64_t md = max_degree(&g) + 1;
   if (nthreads <= 0) nthreads = omp_get_max_threads();
@@ -458,7 +479,7 @@ int or_wcoj_count(uint64_t n, const uint64_t* row_ptrs, const uint32_t* cols, in
     uint32_t path[OR_MAXQ];
 #pragma omp for schedule(dynamic, 64)
     for (int64_t v = v_begin; v < (int64_t)v_end; ++v) {
-      path[0] = (uint32_t)v;
+      path[0] = starts ? starts[v] : (uint32_t)v;
       total += wcoj_dfs(&c, 1, path);
     }
     for (int u = 0; u < qq.nv; ++u) free(c.buf[u]);

@@ -79,6 +79,11 @@ int or_wcoj_count(uint64_t n, const uint64_t* row_ptrs, const uint32_t* cols,
                   int q, int enable_ordering, uint32_t v_begin, uint32_t v_end,
                   int nthreads, uint64_t* out_count);
 
+/* CPU_WCOJ count over an explicit list of start vertices. */
+int or_wcoj_count_list(uint64_t n, const uint64_t* row_ptrs, const uint32_t* cols, int q,
+                       int enable_ordering, const uint32_t* starts, uint64_t nstarts,
+                       int nthreads, uint64_t* out_count);
+
 /* LIGHT-plan evaluation over start vertices [v_begin, v_end) of seq[0].
  * level_out[i] (i < nops) receives the summed output size of exec level i:
  * paths for MATERIALIZE, candidates for COMPUTE / FILTER_COMPUTE /

@@ -78,6 +78,22 @@ def wcoj_count(rp, cols, q, enable_ordering=True, v_begin=0, v_end=None, threads
     return int(out.value)
 
 
+def wcoj_count_list(rp, cols, q, starts, enable_ordering=True, threads=0) -> int:
+    """CPU_WCOJ count over an explicit list of start vertices."""
+    rp, cols = _arrays(rp, cols)
+    starts = np.ascontiguousarray(starts, dtype=np.uint32)
+    L = lib()
+    L.or_wcoj_count_list.argtypes = [ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                     ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int,
+                                     ctypes.POINTER(ctypes.c_uint64)]
+    out = ctypes.c_uint64()
+    rc = L.or_wcoj_count_list(rp.size - 1, rp.ctypes.data, cols.ctypes.data, q, int(enable_ordering),
+                              starts.ctypes.data, starts.size, threads, ctypes.byref(out))
+    if rc:
+        raise ValueError(rc)
+    return int(out.value)
+
+
 def light_count(rp, cols, q, compress_level=3, enable_ordering=True, v_begin=0, v_end=None,
                 threads=0):
     """Returns (count, per-exec-level outputs)."""

@@ -77,6 +77,7 @@ def library() -> ctypes.CDLL:
         "sme_plan_create": ([ctypes.c_int] * 4 + [ctypes.POINTER(vp)], ctypes.c_int),
         "sme_plan_describe": ([vp, ctypes.c_char_p, ctypes.c_size_t], ctypes.c_int),
         "sme_plan_levels": ([vp], ctypes.c_int),
+        "sme_plan_set_reuse": ([vp, ctypes.c_int], ctypes.c_int),
         "sme_plan_destroy": ([vp], None),
         "sme_count": ([vp, vp, vp, ctypes.c_uint32, ctypes.c_uint32, u64p, u64p,
                        ctypes.POINTER(_Stats)], ctypes.c_int),
@@ -198,11 +199,13 @@ def from_adjacency(adj) -> Graph:
 
 class Plan:
     def __init__(self, query: int, algo: int = GPU_LIGHT, compress_level: int = CL_SPECIAL,
-                 enable_ordering: bool = True):
+                 enable_ordering: bool = True, reuse: bool | None = None):
         lib = library()
         h = ctypes.c_void_p()
         _check(lib.sme_plan_create(algo, query, compress_level, int(enable_ordering), ctypes.byref(h)))
         self._h = h
+        if reuse is not None:
+            _check(lib.sme_plan_set_reuse(h, int(reuse)))
         self.query, self.algo, self.compress_level = query, algo, compress_level
         self.enable_ordering = enable_ordering
 

@@ -126,8 +126,9 @@ int sme_plan_create(int algo, int query_type, int compress_level, int enable_ord
   bool ok = false;
   if (algo == SME_ALGO_LIGHT) {
     ok = sme::MakeLightEnginePlan(query_type, compress_level, enable_ordering != 0, &p->plan, &p->text);
-  } else if (algo == SME_ALGO_GPSM) {
-    ok = sme::MakeGpsmEnginePlan(query_type, enable_ordering != 0, &p->plan, &p->text);
+  } else if (algo == SME_ALGO_GPSM || algo == SME_ALGO_GPSM_REUSE) {
+    ok = sme::MakeGpsmEnginePlan(query_type, enable_ordering != 0, algo == SME_ALGO_GPSM_REUSE,
+                                 &p->plan, &p->text);
   } else {
     delete p;
     return Report(nullptr, SME_ERR_UNSUPPORTED, "algorithm " + std::to_string(algo) + " is not supported");
@@ -153,6 +154,12 @@ int sme_plan_describe(const sme_plan* p, char* buf, size_t cap) {
 }
 
 int sme_plan_levels(const sme_plan* p) { return p ? p->plan.nops : -1; }
+
+int sme_plan_set_reuse(sme_plan* p, int on) {
+  if (!p) return SME_ERR_INVALID;
+  p->plan.reuse = on ? 1 : 0;
+  return SME_OK;
+}
 
 void sme_plan_destroy(sme_plan* p) { delete p; }
 

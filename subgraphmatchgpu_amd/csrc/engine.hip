@@ -173,6 +173,9 @@ struct Frame {
   uint32_t matmask = 0;
   CandRef cand[kQ]{};
   uint32_t candmask = 0;
+  // candidate segment of the vertex materialized last, per path (for reuse)
+  CandRef reuse{};
+  bool has_reuse = false;
 };
 
 ColSet SliceCols(const ColSet& c, uint64_t l) {
@@ -203,6 +206,7 @@ Frame SliceFrame(const Frame& f, uint64_t l, uint64_t r) {
   s.cs = SliceCols(f.cs, l);
   for (int w = 0; w < kQ; ++w)
     if (f.candmask & (1u << w)) s.cand[w] = SliceCand(f.cand[w], l);
+  if (f.has_reuse) s.reuse = SliceCand(f.reuse, l);
   return s;
 }
 
@@ -227,7 +231,8 @@ class Runner {
     stats_.d2h_syncs++;
     uint64_t total = host_count_ + dev[0];
     for (int L = 0; L < plan_->nops; ++L) levels_[L] = host_level_[L] + dev[1 + L];
-    if (plan_->algo == kAlgoGpsm) total = levels_[plan_->nops - 1];
+    // GPSM counts the size of the last level (HybridGPSMGPUProcessor.cuh:23-33)
+    if (plan_->algo == kAlgoGpsm || plan_->algo == kAlgoGpsmReuse) total = levels_[plan_->nops - 1];
     *count = total;
     if (levels)
       for (int L = 0; L < plan_->nops; ++L) levels[L] = levels_[L];
@@ -249,10 +254,62 @@ class Runner {
       need_after |= ref;
     }
     std::memset(compute_bconn_, 0, sizeof(compute_bconn_));
+    int compute_level[kQ];
+    for (int w = 0; w < kQ; ++w) compute_level[w] = -1;
     for (int L = 0; L < n; ++L) {
       const OpDesc& op = plan_->ops[L];
-      if ((op.kind == kCompute || op.kind == kComputePathCount) && !compute_bconn_[op.u])
+      if ((op.kind == kCompute || op.kind == kComputePathCount) && !compute_bconn_[op.u]) {
         for (int i = 0; i < op.nb; ++i) compute_bconn_[op.u] |= 1u << op.bconn[i];
+        compute_level[op.u] = L;
+      }
+    }
+    // Intersection reuse.  Level L computing u may start from the candidate
+    // segment of w (materialized at level L-1) when
+    //   - w's set was produced by COMPUTE (values kept, no FILTER since),
+    //   - bconn(w) + {w} is contained in bconn(u),
+    //   - every condition that filtered w's set is implied for u's set:
+    //     non-equality always is (u's compute condition lists every vertex
+    //     materialized before it), and x <op> M[v] holds when u has the same
+    //     condition on v, or on w (then M[w] <op> M[v] gives it).
+    // The reused set then equals the intersection of bconn(w) lists passing
+    // u's conditions, so u's candidates are unchanged.
+    for (int L = 0; L < n; ++L) {
+      reuse_w_[L] = -1;
+      keep_reuse_[L] = false;
+      nrem_[L] = 0;
+    }
+    if (!plan_->reuse) return;
+    for (int L = 2; L < n; ++L) {
+      const OpDesc& op = plan_->ops[L];
+      if (op.kind != kCompute && op.kind != kComputePathCount && op.kind != kComputeCount) continue;
+      const OpDesc& prev = plan_->ops[L - 1];
+      if (prev.kind != kMaterialize) continue;
+      const int w = prev.u;
+      const int Lw = compute_level[w];
+      if (Lw < 0 || plan_->ops[Lw].kind != kCompute) continue;
+      bool filtered = false;
+      for (int k = Lw + 1; k < L; ++k)
+        filtered |= plan_->ops[k].kind == kFilterCompute && plan_->ops[k].u == w;
+      if (filtered) continue;
+      const OpDesc& cw = plan_->ops[Lw];
+      uint32_t bu = 0;
+      for (int i = 0; i < op.nb; ++i) bu |= 1u << op.bconn[i];
+      const uint32_t need = compute_bconn_[w] | (1u << w);
+      if ((need & bu) != need) continue;
+      bool implied = true;
+      for (int i = 0; i < cw.nc && implied; ++i) {
+        const Cond c = cw.cond[i];
+        if (c.op == kNonEqual) continue;
+        bool ok = false;
+        for (int j = 0; j < op.nc; ++j)
+          if (op.cond[j].op == c.op && (op.cond[j].v == c.v || op.cond[j].v == w)) ok = true;
+        implied = ok;
+      }
+      if (!implied) continue;
+      reuse_w_[L] = w;
+      keep_reuse_[L - 1] = true;
+      for (int i = 0; i < op.nb; ++i)
+        if (!(compute_bconn_[w] & (1u << op.bconn[i]))) rem_[L][nrem_[L]++] = op.bconn[i];
     }
   }
 
@@ -439,21 +496,32 @@ class Runner {
   void ComputeLevel(int L, const Frame& f) {
     const OpDesc& op = plan_->ops[L];
     const uint64_t P = f.P;
+    // lists to intersect: the adjacency lists of `lists`, plus the parent's
+    // candidate segment when this level extends it
+    const bool reuse = reuse_w_[L] >= 0 && f.has_reuse;
+    const uint8_t nb = reuse ? nrem_[L] : op.nb;
+    const uint8_t* lists = reuse ? rem_[L] : op.bconn;
+    const int nlists = nb + (reuse ? 1 : 0);
     Buf<uint64_t> size(pool_, P), pstart(pool_, P), offs(pool_, P + 1);
-    Buf<uint8_t> piv(pool_, op.nb > 1 ? P : 0);
+    Buf<uint8_t> piv(pool_, nlists > 1 ? P : 0);
     PrepComputeArgs pa{};
     pa.npaths = P;
     pa.rp = g_->rp;
     pa.cols = g_->cols;
     pa.cs = f.cs;
-    pa.nb = op.nb;
-    std::memcpy(pa.bconn, op.bconn, sizeof(pa.bconn));
+    pa.nb = nb;
+    std::memcpy(pa.bconn, lists, kQ);
+    if (reuse) {
+      pa.has_reuse = 1;
+      pa.reuse = f.reuse;
+    }
     WindowConds(op, &pa.nlt, pa.lt_v, &pa.ngt, pa.gt_v);
     pa.size = size.get();
     pa.pstart = pstart.get();
     pa.piv = piv.get();
     {
-      Timed t(ctx_, st_, kSlotPrep, P * (4.0 * (op.nb + pa.nlt + pa.ngt) + 16.0 * op.nb + 8.0 + 17.0), P);
+      Timed t(ctx_, st_, kSlotPrep,
+              P * (4.0 * (nlists + pa.nlt + pa.ngt) + 16.0 * nlists + 8.0 + 17.0), P);
       LaunchPrepCompute(pa, st_);
     }
     Scan(size.get(), P, offs.get());
@@ -466,7 +534,7 @@ class Runner {
     const bool pair = op.kind == kComputeCount && op.ncun == 1;
     const int u = op.u;
 
-    if (op.nb == 1 && nne == 0 && !pair) {
+    if (nlists == 1 && nne == 0 && !pair) {
       // the candidate set is a window of one adjacency list: no copy
       if (op.kind == kComputeCount) {
         host_level_[L] += W;
@@ -474,6 +542,7 @@ class Runner {
         return;
       }
       Frame f2 = f;
+      f2.has_reuse = false;
       f2.cand[u] = CandRef{g_->cols, pstart.get(), size.get(), nullptr};
       f2.candmask |= 1u << u;
       host_level_[L] += W;
@@ -509,12 +578,16 @@ class Runner {
       ItemArgs it = BaseItems(f, l, r, items, offs.get(), chunk_seg.get(), nchunks);
       it.src = g_->cols;
       it.pstart = pstart.get() + l;
-      it.piv = op.nb > 1 ? piv.get() + l : nullptr;
-      it.nb = op.nb;
-      std::memcpy(it.bconn, op.bconn, sizeof(it.bconn));
+      it.piv = nlists > 1 ? piv.get() + l : nullptr;
+      it.nb = nb;
+      std::memcpy(it.bconn, lists, kQ);
+      if (reuse) {
+        it.has_reuse = 1;
+        it.reuse = SliceCand(f.reuse, l);
+      }
       it.nne = nne;
       std::memcpy(it.ne_v, ne_v, sizeof(it.ne_v));
-      const double seg_bytes = 17.0 + 20.0 * (op.nb - 1) + 4.0 * nne;
+      const double seg_bytes = 17.0 + 20.0 * (nlists - 1) + 4.0 * nne;
       if (op.kind == kComputeCount) {
         if (pair) {
           it.pair = SliceCand(f.cand[op.cun[0]], l);
@@ -522,7 +595,7 @@ class Runner {
         }
         it.acc = acc_;
         it.acc_level = acc_ + 1 + L;
-        Timed t(ctx_, st_, kSlotIntersect, 4.0 * op.nb * items + seg_bytes * nseg + (pair ? 20.0 * nseg : 0), items);
+        Timed t(ctx_, st_, kSlotIntersect, 4.0 * nlists * items + seg_bytes * nseg + (pair ? 20.0 * nseg : 0), items);
         LaunchVerify(it, pair ? kModeSumPair : kModeSum, st_);
         l = r;
         il = ir;
@@ -533,7 +606,7 @@ class Runner {
       it.mask = mask.get();
       it.ccount = ccount.get();
       {
-        Timed t(ctx_, st_, kSlotIntersect, 4.0 * op.nb * items + seg_bytes * nseg + 12.0 * nchunks, items);
+        Timed t(ctx_, st_, kSlotIntersect, 4.0 * nlists * items + seg_bytes * nseg + 12.0 * nchunks, items);
         LaunchVerify(it, kModeMask, st_);
       }
       Buf<uint64_t> cbase(pool_, nchunks + 1);
@@ -555,6 +628,7 @@ class Runner {
       chunk_seg.Reset();
       host_level_[L] += C;
       Frame f2 = SliceFrame(f, l, r);
+      f2.has_reuse = false;
       f2.cand[u] = CandRef{vals.get(), beg.get(), len.get(), nullptr};
       f2.candmask |= 1u << u;
       Next(L, f2);
@@ -597,8 +671,12 @@ class Runner {
     for (int w = 0; w < kQ; ++w)
       if (w != u && (f.candmask & (1u << w))) idx_v[nidx++] = static_cast<uint8_t>(w);
 
+    // the next level extends u's candidate segment: children keep the index
+    // of their parent's segment (one more index column)
+    const bool keep = keep_reuse_[L];
+    const uint8_t nidx_all = static_cast<uint8_t>(nidx + (keep ? 1 : 0));
     // every surviving item becomes a path of the next level
-    const uint64_t bi64 = 64 * (4 * (1 + ncarry + nidx) + NextPathBytes(L)) + kChunkBytes64;
+    const uint64_t bi64 = 64 * (4 * (1 + ncarry + nidx_all) + NextPathBytes(L)) + kChunkBytes64;
     std::vector<uint64_t> ends, item_ends;
     Split(offs.get(), P, W, bi64, 0, L, &ends, &item_ends);
     uint64_t l = 0, il = 0;
@@ -647,10 +725,10 @@ class Runner {
         ea.carry_v[i] = carry_v[i];
         ea.carry_out[i] = carry.back()->get();
       }
-      ea.nidx = nidx;
-      for (int i = 0; i < nidx; ++i) {
+      ea.nidx = nidx_all;
+      for (int i = 0; i < nidx_all; ++i) {
         idx.emplace_back(new Buf<uint32_t>(pool_, C));
-        ea.idx_in[i] = SliceCand(f.cand[idx_v[i]], l);
+        ea.idx_in[i] = SliceCand(i < nidx ? f.cand[idx_v[i]] : cand, l);
         ea.idx_out[i] = idx.back()->get();
       }
       {
@@ -676,6 +754,11 @@ class Runner {
         c.idx = idx[i]->get();
         f2.cand[idx_v[i]] = c;
         f2.candmask |= 1u << idx_v[i];
+      }
+      if (keep) {
+        f2.reuse = ea.idx_in[nidx];
+        f2.reuse.idx = idx[nidx]->get();
+        f2.has_reuse = true;
       }
       Next(L, f2);
       l = r;
@@ -833,6 +916,10 @@ class Runner {
   uint64_t host_count_ = 0;
   uint32_t live_after_[kMaxOps];
   uint32_t compute_bconn_[kQ];
+  int reuse_w_[kMaxOps];         // vertex whose candidate set level L extends
+  bool keep_reuse_[kMaxOps];     // MATERIALIZE at L keeps the parent segment
+  uint8_t rem_[kMaxOps][kQ];     // backward lists the reused set does not cover
+  uint8_t nrem_[kMaxOps];
   RunStats stats_;
 };
 

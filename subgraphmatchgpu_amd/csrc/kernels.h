@@ -39,11 +39,18 @@ struct CandRef {
 // Per-path set-up of an intersection (COMPUTE / COMPUTE_PATH_COUNT /
 // COMPUTE_COUNT).  The pivot is the backward list of smallest degree; it is
 // cut to the value window implied by the < / > conditions.
+//
+// Intersection reuse: when `has_reuse` is set, the parent's candidate segment
+// `reuse` (already the intersection of some of this vertex's backward lists)
+// is one more list; `bconn` then holds only the lists it does not cover.  The
+// pivot id `nb` denotes the reused segment.
 struct PrepComputeArgs {
   uint64_t npaths;
   const uint64_t* rp;
   const uint32_t* cols;
   ColSet cs;
+  CandRef reuse;
+  uint8_t has_reuse;
   uint8_t nb;
   uint8_t bconn[kQ];
   uint8_t nlt, ngt;
@@ -84,6 +91,9 @@ struct ItemArgs {
   const uint8_t* piv;
   uint8_t nb;
   uint8_t bconn[kQ];
+  // reused parent segment (see PrepComputeArgs); piv[s] == nb selects it
+  CandRef reuse;
+  uint8_t has_reuse;
   // non-equality conditions that the window cannot express
   uint8_t nne;
   uint8_t ne_v[kQ];

@@ -100,25 +100,45 @@ __global__ __launch_bounds__(kThreads) void PrepComputeKernel(PrepComputeArgs a)
       const uint64_t v = Gather(a.cs, a.lt_v[i], p);
       hi = v < hi ? v : hi;
     }
-    // pivot: backward list of minimum degree (ThreadChoosePivotLevel,
-    // CheckConstraintsCommon.cuh:10-29; first minimum wins)
+    // pivot: list of minimum length (ThreadChoosePivotLevel,
+    // CheckConstraintsCommon.cuh:10-29; first minimum wins).  A reused
+    // parent segment is short, so it is cut to the window before comparing.
     uint64_t best_b = 0, best_e = 0;
     uint8_t best = 0;
+    bool have = false, windowed = false;
+    if (a.has_reuse) {
+      const uint64_t seg = SegIndex(a.reuse, p);
+      uint64_t b = a.reuse.beg[seg], e = b + a.reuse.len[seg];
+      if (lo >= hi) {
+        e = b;
+      } else {
+        if (lo > 0) b = LowerBound(a.reuse.base, b, e, static_cast<uint32_t>(lo));
+        if (hi <= 0xffffffffull) e = LowerBound(a.reuse.base, b, e, static_cast<uint32_t>(hi));
+      }
+      best = a.nb;
+      best_b = b;
+      best_e = e;
+      have = windowed = true;
+    }
     for (int j = 0; j < a.nb; ++j) {
       const uint32_t v = Gather(a.cs, a.bconn[j], p);
       const uint64_t b = a.rp[v], e = a.rp[v + 1];
-      if (j == 0 || e - b < best_e - best_b) {
+      if (!have || e - b < best_e - best_b) {
         best = static_cast<uint8_t>(j);
         best_b = b;
         best_e = e;
+        have = true;
+        windowed = false;
       }
     }
     uint64_t b = best_b, e = best_e;
-    if (lo >= hi) {
-      e = b;
-    } else {
-      if (lo > 0) b = LowerBound(a.cols, b, e, static_cast<uint32_t>(lo));
-      if (hi <= 0xffffffffull) e = LowerBound(a.cols, b, e, static_cast<uint32_t>(hi));
+    if (!windowed) {
+      if (lo >= hi) {
+        e = b;
+      } else {
+        if (lo > 0) b = LowerBound(a.cols, b, e, static_cast<uint32_t>(lo));
+        if (hi <= 0xffffffffull) e = LowerBound(a.cols, b, e, static_cast<uint32_t>(hi));
+      }
     }
     a.size[p] = e - b;
     a.pstart[p] = b;
@@ -221,17 +241,28 @@ __global__ __launch_bounds__(kThreads) void ChunkPartitionKernel(const uint64_t*
 // GpsmBinSearch, GPSMBinSearch.cuh:146-170, with the < / > part already
 // applied through the window).
 __device__ __forceinline__ bool ItemValid(const ItemArgs& a, uint64_t s, uint32_t x) {
-  if (a.nb > 1) {
+  if (a.nb + a.has_reuse > 1) {
     const uint8_t pv = a.piv[s];
     for (int j = 0; j < a.nb; ++j) {
       if (j == pv) continue;
       const uint32_t v = Gather(a.cs, a.bconn[j], s);
       if (!Contains(a.cols, a.rp[v], a.rp[v + 1], x)) return false;
     }
+    if (a.has_reuse && pv != a.nb) {
+      const uint64_t seg = SegIndex(a.reuse, s);
+      const uint64_t b = a.reuse.beg[seg];
+      if (!Contains(a.reuse.base, b, b + a.reuse.len[seg], x)) return false;
+    }
   }
   for (int i = 0; i < a.nne; ++i)
     if (x == Gather(a.cs, a.ne_v[i], s)) return false;
   return true;
+}
+
+// Values of segment s come from the graph's lists or, when the pivot is the
+// reused parent segment, from that candidate buffer.
+__device__ __forceinline__ const uint32_t* ItemSrc(const ItemArgs& a, uint64_t s) {
+  return (a.has_reuse && a.piv[s] == a.nb) ? a.reuse.base : a.src;
 }
 
 template <int kMode>
@@ -270,7 +301,7 @@ __global__ __launch_bounds__(kThreads) void VerifyKernel(ItemArgs a) {
         local += add;
         continue;
       }
-      x = a.src[a.pstart[s] + r];
+      x = ItemSrc(a, s)[a.pstart[s] + r];
       ok = ItemValid(a, s, x);
     }
     if (kMode == kModeMask) {
@@ -318,7 +349,7 @@ __global__ __launch_bounds__(kThreads) void CompactValuesKernel(CompactArgs a) {
     const uint64_t s = SegmentOf(it, c, t);
     const uint64_t r = it.offs[0] + t - it.offs[s];
     const uint64_t pos = a.rk.mask ? a.rk.cbase[c] + __popcll(m & __lanemask_lt()) : t;
-    a.out_vals[pos] = it.src[it.pstart[s] + r];
+    a.out_vals[pos] = ItemSrc(it, s)[it.pstart[s] + r];
   }
 }
 

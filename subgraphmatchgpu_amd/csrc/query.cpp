@@ -361,6 +361,7 @@ bool MakeLightEnginePlan(int query_type, int compress_level, bool enable_orderin
   out->nv = q.VertexCount();
   out->nops = static_cast<int>(ops.size());
   out->direct_count = (compress_level == kClSpecial && CanDirectCount(query_type)) ? 1 : 0;
+  out->reuse = 1;
   for (size_t i = 0; i < ops.size(); ++i) {
     OpDesc& d = out->ops[i];
     d.kind = ops[i].op;
@@ -392,14 +393,15 @@ bool MakeLightEnginePlan(int query_type, int compress_level, bool enable_orderin
   return true;
 }
 
-bool MakeGpsmEnginePlan(int query_type, bool enable_ordering, EnginePlan* out,
+bool MakeGpsmEnginePlan(int query_type, bool enable_ordering, bool reuse, EnginePlan* out,
                         std::string* describe) {
   Query q;
   if (!q.Init(query_type, enable_ordering)) return false;
   const int nv = q.VertexCount();
   if (2 * nv > kMaxOps) return false;
   *out = EnginePlan{};
-  out->algo = kAlgoGpsm;
+  out->algo = reuse ? kAlgoGpsmReuse : kAlgoGpsm;
+  out->reuse = reuse ? 1 : 0;
   out->query_type = query_type;
   out->compress_level = -1;
   out->enable_ordering = enable_ordering ? 1 : 0;

@@ -134,6 +134,7 @@ struct EnginePlan {
   int nv;
   int nops;
   int direct_count;    // COUNT uses the per-path closed form (LIGHTDirectCount)
+  int reuse;           // extend the parent's candidate set where provably equal
   OpDesc ops[kMaxOps];
 };
 
@@ -142,8 +143,10 @@ bool MakeLightEnginePlan(int query_type, int compress_level, bool enable_orderin
                          EnginePlan* out, std::string* describe);
 // GPSM: identity order, GetOrderedConnectivity + GetWholeOrderedOrdering,
 // every level computed then materialized; count = size of the last level
-// (src/gpu/impl/gpsm/HybridGPSMGPUProcessor.cuh:23-33).
-bool MakeGpsmEnginePlan(int query_type, bool enable_ordering, EnginePlan* out,
+// (src/gpu/impl/gpsm/HybridGPSMGPUProcessor.cuh:23-33).  With `reuse` the
+// plan is GPSM_REUSE: a level extends the previous level's intersection
+// where that is provably the same set (src/gpu/impl/gpsm/reuse).
+bool MakeGpsmEnginePlan(int query_type, bool enable_ordering, bool reuse, EnginePlan* out,
                         std::string* describe);
 
 // CanDirectCount (src/gpu/impl/light/op/LIGHTDirectCount.cuh:13-19).

@@ -58,10 +58,11 @@ def test_emu_light_all_compress_levels(emu, graphs, q):
 def test_emu_gpsm(emu, graphs, q):
     g, p = graphs["rmat7"]
     want, wl = oracle.gpsm_count(g.row_ptrs, g.cols, q)
-    cnt, lv, _ = run_emu(emu, p, 4, q, 0, 0, g.n, 1 << 30)
-    assert cnt == want
-    # the engine's levels alternate C/M; the M levels are GPSM's level sizes
-    assert [lv[0]] + lv[2::2] == wl[:len(lv[2::2]) + 1]
+    for algo in (4, 5):  # GPU_GPSM, GPU_GPSM_REUSE
+        cnt, lv, _ = run_emu(emu, p, algo, q, 0, 0, g.n, 1 << 30)
+        assert cnt == want
+        # the engine's levels alternate C/M; the M levels are GPSM's level sizes
+        assert [lv[0]] + lv[2::2] == wl[:len(lv[2::2]) + 1]
 
 
 def test_emu_tiny_budget_invariance(emu, graphs):

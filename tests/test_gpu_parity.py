@@ -45,11 +45,23 @@ def test_light_all_compress_levels(dev, small, q):
 @pytest.mark.parametrize("q", list(range(14)))
 def test_gpsm(dev, small, q):
     g, dg = small["rmat8"]
-    say(f"case gpsm q={q}")
     want, wl = oracle.gpsm_count(g.row_ptrs, g.cols, q)
-    res = dev.count(dg, sme.Plan(q, sme.GPU_GPSM))
-    assert res.count == want
-    assert [res.levels[0]] + res.levels[2::2] == wl[:len(res.levels[2::2]) + 1]
+    for algo in (sme.GPU_GPSM, sme.GPU_GPSM_REUSE):
+        say(f"case gpsm q={q} algo={algo}")
+        res = dev.count(dg, sme.Plan(q, algo))
+        assert res.count == want
+        assert [res.levels[0]] + res.levels[2::2] == wl[:len(res.levels[2::2]) + 1]
+
+
+@pytest.mark.parametrize("q", [0, 1, 2, 3, 4, 6, 7, 11, 12])
+def test_light_reuse_on_off(dev, small, q):
+    """Intersection reuse changes the work, never the result."""
+    g, dg = small["rmat10"]
+    say(f"case reuse q={q}")
+    a = dev.count(dg, sme.Plan(q, reuse=True))
+    b = dev.count(dg, sme.Plan(q, reuse=False))
+    assert (a.count, a.levels) == (b.count, b.levels)
+    assert (a.count, a.levels) == oracle.light_count(g.row_ptrs, g.cols, q, 3)
 
 
 def test_golden_reference_counts(dev, ref_counts, fixture_graphs):
