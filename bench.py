@@ -71,6 +71,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--budget-gb", type=float, default=64.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="first run one device copy of this size (counter calibration)")
+    ap.add_argument("--hub-min-degree", type=int, default=0,
+                    help="adjacency bitmaps for vertices of at least this degree (0: off)")
+    ap.add_argument("--hub-max-gb", type=float, default=8.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU; --gpus must be 1"}))
@@ -82,7 +87,15 @@ def main():
     g = sme.gen_rmat(SCALE, EDGE_FACTOR, 0.57, 0.19, 0.19, SEED)
     log(f"graph rmat{SCALE} ef{EDGE_FACTOR}: n={g.n} nnz={g.nnz} ({time.perf_counter() - t0:.1f}s)")
     dev = sme.Device(0, budget_bytes=int(args.budget_gb * (1 << 30)))
+    if args.calib_copy_mib:
+        ms = dev.calibrate_copy(args.calib_copy_mib << 20)
+        log(f"calibration copy of {args.calib_copy_mib} MiB: {ms:.3f} ms "
+            f"({2 * (args.calib_copy_mib << 20) / ms / 1e6:.1f} GB/s read+write)")
+    if args.hub_min_degree:
+        dev.set_hub_bitmaps(args.hub_min_degree, int(args.hub_max_gb * (1 << 30)))
+    t0 = time.perf_counter()
     dg = dev.upload(g)
+    log(f"upload: {1e3 * (time.perf_counter() - t0):.1f} ms, {dg.hubs} hub bitmaps")
     plans = [sme.Plan(q, sme.GPU_LIGHT, sme.CL_SPECIAL) for q in QUERIES]
 
     def step():
@@ -92,7 +105,8 @@ def main():
             r = dev.count(dg, p)
             counts.append(r.count)
             log(f"  Q{q}: {r.count} embeddings in {1e3 * (time.perf_counter() - t):.1f} ms, "
-                f"{r.batches} batches, {r.d2h_syncs} syncs, peak pool {r.peak_pool_bytes / 2**30:.2f} GiB")
+                f"{r.batches} batches, {r.d2h_syncs} syncs, peak pool {r.peak_pool_bytes / 2**30:.2f} GiB, "
+                f"levels {r.levels}")
         return counts
 
     for i in range(args.warmup):
@@ -137,9 +151,11 @@ def main():
                         "4-clique then diamond, LIGHT, all start vertices",
             "counts": {"Q3": counts[0], "Q2": counts[1]},
             "budget_gb": args.budget_gb,
+            "hub_bitmaps": {"min_degree": args.hub_min_degree, "rows": dg.hubs},
             "kernel_ms_per_step": kernel_ms,
             "kernel_families": {k: {"launches": v["launches"], "ms": round(v["ms"], 3),
-                                    "GB": round(v["bytes"] / 1e9, 3)} for k, v in prof.items()},
+                                    "GB": round(v["bytes"] / 1e9, 3), "items": v["items"]}
+                                for k, v in prof.items()},
         },
         "roofline": {
             "bound": "hbm",

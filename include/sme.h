@@ -94,6 +94,14 @@ int sme_device_count(int* out);
 int sme_graph_upload(sme_ctx* ctx, const uint64_t* row_ptrs, const uint32_t* cols, uint64_t n,
                      uint64_t nnz, sme_graph** out);
 void sme_graph_destroy(sme_graph* g);
+/* Adjacency bitmaps for high-degree vertices, built by later uploads: every
+ * vertex of degree >= min_degree, highest degree first, within max_bytes of
+ * device memory (one bit per vertex id per row).  Membership tests against
+ * those lists become one bit test instead of a binary search.  0 disables
+ * (the default).  No counterpart in the reference, which always
+ * binary-searches (gpu/utils/ops/GPUBinSearch.cuh:18-31). */
+int sme_set_hub_bitmaps(sme_ctx* ctx, uint64_t min_degree, uint64_t max_bytes);
+int sme_graph_hubs(const sme_graph* g, uint64_t* nhub);
 
 /* ---- plan (host only) ------------------------------------------------- */
 int sme_plan_create(int algo, int query_type, int compress_level, int enable_ordering,
@@ -126,6 +134,11 @@ int sme_profile_slots(void);
 /* Per kernel family: name, launches, summed ms, algorithmic bytes, items. */
 int sme_profile_get(const sme_ctx* ctx, int slot, char* name, size_t cap, uint64_t* launches,
                     double* ms, double* bytes, uint64_t* items);
+
+/* Diagnostic with no counterpart in the reference: copies `bytes` within the
+ * context's pool with one device-to-device copy and returns its time.  Used
+ * to calibrate hardware traffic counters against a copy of known size. */
+int sme_calibrate_copy(sme_ctx* ctx, uint64_t bytes, double* ms);
 
 /* ---- host graph utilities (no device needed) --------------------------- */
 /* Arrays returned through row_ptrs / cols are released with sme_free_host. */

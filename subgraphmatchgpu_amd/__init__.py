@@ -98,6 +98,9 @@ def library() -> ctypes.CDLL:
         "sme_gen_er": ([ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, u64p, u64p,
                         ctypes.POINTER(u64p), ctypes.POINTER(u32p)], ctypes.c_int),
         "sme_free_host": ([vp], None),
+        "sme_calibrate_copy": ([vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+        "sme_set_hub_bitmaps": ([vp, ctypes.c_uint64, ctypes.c_uint64], ctypes.c_int),
+        "sme_graph_hubs": ([vp, u64p], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -280,6 +283,17 @@ class Device:
                                         "items": it.value}
         return out
 
+    def set_hub_bitmaps(self, min_degree: int, max_bytes: int):
+        """Adjacency bitmaps for vertices of degree >= min_degree in graphs
+        uploaded afterwards (0 disables)."""
+        _check(library().sme_set_hub_bitmaps(self._h, min_degree, max_bytes), self._h)
+
+    def calibrate_copy(self, nbytes: int) -> float:
+        """One device-to-device copy of nbytes inside the pool; returns ms."""
+        ms = ctypes.c_double()
+        _check(library().sme_calibrate_copy(self._h, nbytes, ctypes.byref(ms)), self._h)
+        return ms.value
+
     def close(self):
         if getattr(self, "_h", None):
             for g in self._graphs:
@@ -306,6 +320,12 @@ class DeviceGraph:
         self._h = h
         self.n, self.nnz = g.n, g.nnz
         dev._graphs.append(self)
+
+    @property
+    def hubs(self) -> int:
+        c = ctypes.c_uint64()
+        _check(library().sme_graph_hubs(self._h, ctypes.byref(c)))
+        return c.value
 
     def release(self):
         if getattr(self, "_h", None):

@@ -180,6 +180,19 @@ int sme_count(sme_ctx* ctx, const sme_graph* g, const sme_plan* p, uint32_t v_be
   return SME_OK;
 }
 
+int sme_set_hub_bitmaps(sme_ctx* ctx, uint64_t min_degree, uint64_t max_bytes) {
+  if (!ctx) return SME_ERR_INVALID;
+  ctx->impl.hub_min_degree = min_degree;
+  ctx->impl.hub_max_bytes = max_bytes;
+  return SME_OK;
+}
+
+int sme_graph_hubs(const sme_graph* g, uint64_t* nhub) {
+  if (!g || !nhub) return SME_ERR_INVALID;
+  *nhub = g->impl.nhub;
+  return SME_OK;
+}
+
 int sme_profile_enable(sme_ctx* ctx, int on) {
   if (!ctx) return SME_ERR_INVALID;
   ctx->impl.profiling = on != 0;
@@ -207,6 +220,35 @@ int sme_profile_get(const sme_ctx* ctx, int slot, char* name, size_t cap, uint64
   if (bytes) *bytes = s.bytes;
   if (items) *items = s.items;
   return SME_OK;
+}
+
+int sme_calibrate_copy(sme_ctx* ctx, uint64_t bytes, double* ms) {
+  if (!ctx || !ms || bytes == 0) return Report(ctx, SME_ERR_INVALID, "bad argument");
+  sme::DevicePool& pool = ctx->impl.pool();
+  void* a = pool.Alloc(bytes);
+  void* b = a ? pool.Alloc(bytes) : nullptr;
+  if (!a || !b) {
+    pool.Free(a);
+    return Report(ctx, SME_ERR_OOM, "pool too small for the copy");
+  }
+  hipStream_t st = ctx->impl.stream();
+  hipEvent_t e0, e1;
+  int rc = SME_OK;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess ||
+      hipMemsetAsync(a, 1, bytes, st) != hipSuccess || hipEventRecord(e0, st) != hipSuccess ||
+      hipMemcpyAsync(b, a, bytes, hipMemcpyDeviceToDevice, st) != hipSuccess ||
+      hipEventRecord(e1, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess) {
+    rc = SME_ERR_HIP;
+  } else {
+    float f = 0;
+    (void)hipEventElapsedTime(&f, e0, e1);
+    *ms = f;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  pool.Free(b);
+  pool.Free(a);
+  return rc == SME_OK ? SME_OK : Report(ctx, rc, "copy failed");
 }
 
 int sme_bin_read(const char* path, uint64_t* n, uint64_t* nnz, uint64_t** row_ptrs, uint32_t** cols) {

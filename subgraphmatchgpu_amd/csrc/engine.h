@@ -61,6 +61,12 @@ class Context {
   DevicePool& pool() { return pool_; }
   uint64_t* pinned() { return pinned_; }
 
+  // hub adjacency bitmaps for graphs uploaded from now on: vertices of
+  // degree >= hub_min_degree, highest degree first, at most hub_max_bytes
+  // (0 disables)
+  uint64_t hub_min_degree = 0;
+  uint64_t hub_max_bytes = 0;
+
   // profiling
   bool profiling = false;
   std::array<SlotStats, kNumSlots> slots{};
@@ -96,6 +102,10 @@ class DevGraph {
   uint32_t* cols = nullptr;
   bool self_loops = false;
   int device = -1;
+  // hub adjacency bitmaps (nhub rows of hub_words u32 words)
+  uint32_t* hub_id = nullptr;
+  uint32_t* hub_bits = nullptr;
+  uint64_t nhub = 0, hub_words = 0;
 };
 
 // Runs one plan over start vertices [v_begin, v_end) of the first matched

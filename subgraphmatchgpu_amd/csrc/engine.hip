@@ -102,6 +102,8 @@ DevGraph::~DevGraph() {
   if (device >= 0) (void)hipSetDevice(device);
   if (rp) (void)hipFree(rp);
   if (cols) (void)hipFree(cols);
+  if (hub_id) (void)hipFree(hub_id);
+  if (hub_bits) (void)hipFree(hub_bits);
 }
 
 Status DevGraph::Upload(Context* ctx, const uint64_t* row_ptrs, const uint32_t* c, uint64_t nv,
@@ -131,6 +133,39 @@ Status DevGraph::Upload(Context* ctx, const uint64_t* row_ptrs, const uint32_t* 
   if (ne)
     SME_HIP(hipMemcpyAsync(cols, c, ne * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream()));
   SME_HIP(hipStreamSynchronize(ctx->stream()));
+  if (ctx->hub_min_degree && ctx->hub_max_bytes && nv) {
+    // highest-degree vertices first, as many rows as the byte cap allows
+    hub_words = (nv + 31) / 32;
+    std::vector<uint32_t> cand;
+    for (uint64_t v = 0; v < nv; ++v)
+      if (row_ptrs[v + 1] - row_ptrs[v] >= ctx->hub_min_degree) cand.push_back(static_cast<uint32_t>(v));
+    std::sort(cand.begin(), cand.end(), [&](uint32_t a, uint32_t b) {
+      const uint64_t da = row_ptrs[a + 1] - row_ptrs[a], db = row_ptrs[b + 1] - row_ptrs[b];
+      return da != db ? da > db : a < b;
+    });
+    const uint64_t row_bytes = hub_words * sizeof(uint32_t);
+    nhub = std::min<uint64_t>(cand.size(), ctx->hub_max_bytes / row_bytes);
+    cand.resize(nhub);
+    if (nhub) {
+      std::vector<uint32_t> ids(nv, kNoHub);
+      for (uint64_t h = 0; h < nhub; ++h) ids[cand[h]] = static_cast<uint32_t>(h);
+      uint32_t* hubs = nullptr;
+      if (hipMalloc(reinterpret_cast<void**>(&hub_id), nv * sizeof(uint32_t)) != hipSuccess ||
+          hipMalloc(reinterpret_cast<void**>(&hub_bits), nhub * row_bytes) != hipSuccess ||
+          hipMalloc(reinterpret_cast<void**>(&hubs), nhub * sizeof(uint32_t)) != hipSuccess) {
+        (void)hipGetLastError();
+        if (hubs) (void)hipFree(hubs);
+        *err = "hipMalloc of the hub bitmaps failed";
+        return kErrOom;
+      }
+      SME_HIP(hipMemcpyAsync(hub_id, ids.data(), nv * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream()));
+      SME_HIP(hipMemcpyAsync(hubs, cand.data(), nhub * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream()));
+      LaunchFill32(hub_bits, nhub * hub_words, 0, ctx->stream());
+      LaunchBuildHubBits(rp, cols, hubs, nhub, hub_words, hub_bits, ctx->stream());
+      SME_HIP(hipStreamSynchronize(ctx->stream()));
+      (void)hipFree(hubs);
+    }
+  }
   return kOk;
 }
 
@@ -277,6 +312,23 @@ class Runner {
       reuse_w_[L] = -1;
       keep_reuse_[L] = false;
       nrem_[L] = 0;
+      alias_of_[L] = -1;
+    }
+    // Identical intersections in one frame (e.g. the diamond's CPC1 and
+    // CPC3, both N(v0) and N(v2) with only implied non-equalities): the
+    // later level takes the earlier level's candidate set.
+    for (int L = 1; L < n; ++L) {
+      const OpDesc& op = plan_->ops[L];
+      if (op.kind != kCompute && op.kind != kComputePathCount) continue;
+      for (int Lp = L - 1; Lp >= 1; --Lp) {
+        const OpDesc& pr = plan_->ops[Lp];
+        if (pr.kind != kCompute && pr.kind != kComputePathCount) break;  // frame changed
+        if (op.kind == kCompute && pr.kind != kCompute) continue;       // values needed
+        if (SameIntersection(op, pr)) {
+          alias_of_[L] = Lp;
+          break;
+        }
+      }
     }
     if (!plan_->reuse) return;
     for (int L = 2; L < n; ++L) {
@@ -311,6 +363,28 @@ class Runner {
       for (int i = 0; i < op.nb; ++i)
         if (!(compute_bconn_[w] & (1u << op.bconn[i]))) rem_[L][nrem_[L]++] = op.bconn[i];
     }
+  }
+
+  // Same backward lists and the same effective conditions (non-equalities
+  // implied by list membership dropped).
+  bool SameIntersection(const OpDesc& a, const OpDesc& b) const {
+    uint32_t ba = 0, bb = 0;
+    for (int i = 0; i < a.nb; ++i) ba |= 1u << a.bconn[i];
+    for (int i = 0; i < b.nb; ++i) bb |= 1u << b.bconn[i];
+    if (ba != bb) return false;
+    auto effective = [&](const OpDesc& o, uint32_t* lt, uint32_t* gt, uint32_t* ne) {
+      *lt = *gt = *ne = 0;
+      for (int i = 0; i < o.nc; ++i) {
+        const uint32_t bit = 1u << o.cond[i].v;
+        if (o.cond[i].op == kLess) *lt |= bit;
+        if (o.cond[i].op == kLarger) *gt |= bit;
+        if (o.cond[i].op == kNonEqual && (g_->self_loops || !(ba & bit))) *ne |= bit;
+      }
+    };
+    uint32_t l1, g1, n1, l2, g2, n2;
+    effective(a, &l1, &g1, &n1);
+    effective(b, &l2, &g2, &n2);
+    return l1 == l2 && g1 == g2 && n1 == n2;
   }
 
   uint64_t ReadU64(const uint64_t* dptr) {
@@ -487,6 +561,9 @@ class Runner {
     it.offs = offs + l;
     it.chunk_seg = chunk_seg;
     it.rp = g_->rp;
+    it.hub_id = g_->nhub ? g_->hub_id : nullptr;
+    it.hub_bits = g_->hub_bits;
+    it.hub_words = g_->hub_words;
     it.cols = g_->cols;
     it.cs = SliceCols(f.cs, l);
     return it;
@@ -496,6 +573,27 @@ class Runner {
   void ComputeLevel(int L, const Frame& f) {
     const OpDesc& op = plan_->ops[L];
     const uint64_t P = f.P;
+    if (alias_of_[L] >= 0 && (f.candmask & (1u << plan_->ops[alias_of_[L]].u))) {
+      // same set as an earlier level of this frame: share it
+      Frame f2 = f;
+      f2.has_reuse = false;
+      f2.cand[op.u] = f.cand[plan_->ops[alias_of_[L]].u];
+      f2.candmask |= 1u << op.u;
+      CountPathArgs ca{};
+      ca.npaths = P;
+      ca.mode = 1;
+      ca.ncun = 1;
+      ca.cun[0] = op.u;
+      ca.cand[op.u] = f2.cand[op.u];
+      ca.acc_level = acc_ + 1 + L;
+      {
+        Timed t(ctx_, st_, kSlotCount, P * 12.0, P);
+        LaunchCountPaths(ca, st_);
+      }
+      stats_.batches++;
+      Next(L, f2);
+      return;
+    }
     // lists to intersect: the adjacency lists of `lists`, plus the parent's
     // candidate segment when this level extends it
     const bool reuse = reuse_w_[L] >= 0 && f.has_reuse;
@@ -916,6 +1014,7 @@ class Runner {
   uint64_t host_count_ = 0;
   uint32_t live_after_[kMaxOps];
   uint32_t compute_bconn_[kQ];
+  int alias_of_[kMaxOps];        // earlier level computing the same set, or -1
   int reuse_w_[kMaxOps];         // vertex whose candidate set level L extends
   bool keep_reuse_[kMaxOps];     // MATERIALIZE at L keeps the parent segment
   uint8_t rem_[kMaxOps][kQ];     // backward lists the reused set does not cover

@@ -94,6 +94,11 @@ struct ItemArgs {
   // reused parent segment (see PrepComputeArgs); piv[s] == nb selects it
   CandRef reuse;
   uint8_t has_reuse;
+  // adjacency bitmaps of high-degree vertices: membership of x in N(v) is
+  // one bit when hub_id[v] != kNoHub (hub_id == nullptr: none built)
+  const uint32_t* hub_id;
+  const uint32_t* hub_bits;
+  uint64_t hub_words;
   // non-equality conditions that the window cannot express
   uint8_t nne;
   uint8_t ne_v[kQ];
@@ -176,6 +181,11 @@ void LaunchChunkPartition(const uint64_t* offs, uint64_t nseg, uint64_t nchunks,
 void LaunchVerify(const ItemArgs& a, int mode, hipStream_t stream);
 void LaunchCompact(const CompactArgs& a, hipStream_t stream);
 void LaunchExpand(const ExpandArgs& a, hipStream_t stream);
+constexpr uint32_t kNoHub = 0xffffffffu;
+// Sets bit x of row h for every x in N(hubs[h]); rows must be zero.
+void LaunchBuildHubBits(const uint64_t* rp, const uint32_t* cols, const uint32_t* hubs,
+                        uint64_t nhub, uint64_t words, uint32_t* bits, hipStream_t stream);
+void LaunchFill32(uint32_t* out, uint64_t n, uint32_t value, hipStream_t stream);
 void LaunchIota32(uint32_t* out, uint64_t n, uint32_t base, hipStream_t stream);
 void LaunchFill64(uint64_t* out, uint64_t n, uint64_t value, hipStream_t stream);
 // Greedy batch split over paths [start, npaths): writes up to max_out pairs

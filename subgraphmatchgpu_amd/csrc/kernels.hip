@@ -218,8 +218,8 @@ __global__ __launch_bounds__(kThreads) void CountPathsKernel(CountPathArgs a) {
   }
   local = WaveSum(local);
   if ((threadIdx.x & (kWave - 1)) == 0 && local) {
-    atomicAdd(a.acc, local);
-    atomicAdd(a.acc_level, local);
+    if (a.acc) atomicAdd(a.acc, local);
+    if (a.acc_level) atomicAdd(a.acc_level, local);
   }
 }
 
@@ -246,7 +246,12 @@ __device__ __forceinline__ bool ItemValid(const ItemArgs& a, uint64_t s, uint32_
     for (int j = 0; j < a.nb; ++j) {
       if (j == pv) continue;
       const uint32_t v = Gather(a.cs, a.bconn[j], s);
-      if (!Contains(a.cols, a.rp[v], a.rp[v + 1], x)) return false;
+      const uint32_t h = a.hub_id ? a.hub_id[v] : kNoHub;
+      if (h != kNoHub) {
+        if (!((a.hub_bits[h * a.hub_words + (x >> 5)] >> (x & 31)) & 1u)) return false;
+      } else if (!Contains(a.cols, a.rp[v], a.rp[v + 1], x)) {
+        return false;
+      }
     }
     if (a.has_reuse && pv != a.nb) {
       const uint64_t seg = SegIndex(a.reuse, s);
@@ -383,6 +388,26 @@ __global__ __launch_bounds__(kThreads) void ExpandKernel(ExpandArgs a) {
     for (int i = 0; i < a.nidx; ++i)
       a.idx_out[i][pos] = static_cast<uint32_t>(SegIndex(a.idx_in[i], s));
   }
+}
+
+// One work-group per hub: OR each neighbour's bit into the hub's row.
+__global__ __launch_bounds__(kThreads) void BuildHubBitsKernel(const uint64_t* rp, const uint32_t* cols,
+                                                               const uint32_t* hubs, uint64_t nhub,
+                                                               uint64_t words, uint32_t* bits) {
+  for (uint64_t h = blockIdx.x; h < nhub; h += gridDim.x) {
+    const uint32_t v = hubs[h];
+    uint32_t* row = bits + h * words;
+    for (uint64_t j = rp[v] + threadIdx.x; j < rp[v + 1]; j += blockDim.x) {
+      const uint32_t x = cols[j];
+      atomicOr(row + (x >> 5), 1u << (x & 31));
+    }
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void Fill32Kernel(uint32_t* out, uint64_t n, uint32_t v) {
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += GridStride())
+    out[i] = v;
 }
 
 __global__ __launch_bounds__(kThreads) void Iota32Kernel(uint32_t* out, uint64_t n, uint32_t base) {
@@ -574,6 +599,18 @@ void LaunchCompact(const CompactArgs& a, hipStream_t stream) {
 void LaunchExpand(const ExpandArgs& a, hipStream_t stream) {
   if (a.it.nchunks == 0) return;
   hipLaunchKernelGGL(ExpandKernel, dim3(GridFor(a.it.nchunks, kWavesPerBlock)), dim3(kThreads), 0, stream, a);
+}
+
+void LaunchBuildHubBits(const uint64_t* rp, const uint32_t* cols, const uint32_t* hubs, uint64_t nhub,
+                        uint64_t words, uint32_t* bits, hipStream_t stream) {
+  if (nhub == 0) return;
+  hipLaunchKernelGGL(BuildHubBitsKernel, dim3(GridFor(nhub, 1)), dim3(kThreads), 0, stream, rp, cols, hubs,
+                     nhub, words, bits);
+}
+
+void LaunchFill32(uint32_t* out, uint64_t n, uint32_t value, hipStream_t stream) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(Fill32Kernel, dim3(GridFor(n, kThreads)), dim3(kThreads), 0, stream, out, n, value);
 }
 
 void LaunchIota32(uint32_t* out, uint64_t n, uint32_t base, hipStream_t stream) {

@@ -8,8 +8,9 @@
 #include "../../include/sme.h"
 
 int main(int argc, char** argv) {
-  if (argc != 9) {
-    std::fprintf(stderr, "usage: %s graph.bin algo query cl ord vbegin vend budget\n", argv[0]);
+  if (argc != 9 && argc != 10) {
+    std::fprintf(stderr, "usage: %s graph.bin algo query cl ord vbegin vend budget [hub_min_degree]\n",
+                 argv[0]);
     return 2;
   }
   uint64_t n = 0, nnz = 0;
@@ -30,6 +31,7 @@ int main(int argc, char** argv) {
     std::fprintf(stderr, "open: %s\n", sme_last_error(nullptr));
     return 1;
   }
+  if (argc == 10) sme_set_hub_bitmaps(ctx, std::strtoull(argv[9], nullptr, 10), 1ull << 30);
   sme_plan* plan = nullptr;
   if (sme_plan_create(algo, q, cl, ord, &plan) != SME_OK) {
     std::fprintf(stderr, "plan: %s\n", sme_last_error(nullptr));

@@ -107,6 +107,11 @@ inline unsigned long long __lanemask_lt() {
   return lane ? (~0ull >> (64 - lane)) : 0ull;
 }
 inline void __syncthreads() { emu::BlockBarrier(); }
+inline unsigned int atomicOr(unsigned int* p, unsigned int v) {
+  const unsigned int o = *p;
+  *p = o | v;
+  return o;
+}
 inline unsigned long long atomicAdd(unsigned long long* p, unsigned long long v) {
   const unsigned long long o = *p;
   *p = o + v;

@@ -21,10 +21,12 @@ def emu():
     return EMU
 
 
-def run_emu(emu, path, algo, q, cl, vb, ve, budget, ordering=1):
+def run_emu(emu, path, algo, q, cl, vb, ve, budget, ordering=1, hub_min_degree=None):
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0")
-    r = subprocess.run([emu, path, str(algo), str(q), str(cl), str(ordering), str(vb), str(ve),
-                        str(budget)], capture_output=True, text=True, env=env, timeout=600)
+    cmd = [emu, path, str(algo), str(q), str(cl), str(ordering), str(vb), str(ve), str(budget)]
+    if hub_min_degree is not None:
+        cmd.append(str(hub_min_degree))
+    r = subprocess.run(cmd, capture_output=True, text=True, env=env, timeout=600)
     assert r.returncode == 0, r.stderr[-3000:]
     parts = r.stdout.split()
     cnt = int(parts[0].split("=")[1])
@@ -83,6 +85,16 @@ def test_emu_ranges_and_empty(emu, graphs):
         for a, b in ((0, 0), (5, 5), (0, 17), (17, 90), (90, g.n)):
             want = oracle.light_count(g.row_ptrs, g.cols, q, 3, v_begin=a, v_end=b)
             assert run_emu(emu, p, 6, q, 3, a, b, 1 << 30)[:2] == (want[0], want[1])
+
+
+def test_emu_hub_bitmaps(emu, graphs):
+    """Membership through hub bitmaps (low threshold: most lists are hubs)."""
+    g, p = graphs["rmat7"]
+    for q in (0, 1, 2, 3, 4, 7, 13):
+        want = oracle.light_count(g.row_ptrs, g.cols, q, 3)
+        for thr in (1, 8):
+            assert run_emu(emu, p, 6, q, 3, 0, g.n, 1 << 30, hub_min_degree=thr)[:2] == \
+                (want[0], want[1]), (q, thr)
 
 
 def test_emu_clique_without_ordering(emu, graphs):

@@ -139,6 +139,21 @@ def test_start_ranges(dev, small):
     assert dev.count(dg, sme.Plan(0), 10, 3).count == 0
 
 
+def test_hub_bitmaps(fixture_graphs):
+    """Membership through adjacency bitmaps gives the same counts and levels."""
+    g = fixture_graphs["rmat10"]
+    d = sme.Device(0, budget_bytes=4 << 30)
+    d.set_hub_bitmaps(16, 1 << 30)
+    dg = d.upload(g)
+    assert dg.hubs > 0
+    for q in (0, 1, 2, 3, 4, 7):
+        say(f"case hubs q={q}")
+        want = oracle.light_count(g.row_ptrs, g.cols, q, 3)
+        r = d.count(dg, sme.Plan(q))
+        assert (r.count, r.levels) == want
+    d.close()
+
+
 def test_edge_graphs(dev):
     empty = sme.from_adjacency([[] for _ in range(10)])
     dg = dev.upload(empty)
