@@ -127,6 +127,16 @@ void sme_plan_destroy(sme_plan* p);
 int sme_count(sme_ctx* ctx, const sme_graph* g, const sme_plan* p, uint32_t v_begin,
               uint32_t v_end, uint64_t* out_count, uint64_t* level_counts, sme_stats* stats);
 
+/* One enumeration over [bounds[0], bounds[nbuckets]) that also splits the
+ * count by start vertex: counts[k] receives the embeddings whose first
+ * matched vertex lies in [bounds[k], bounds[k+1]) (bounds non-decreasing).
+ * Used for graphs packed back to back (one bucket per graph) and for merged
+ * requests over one resident graph.  No counterpart in the reference, which
+ * sums one count per task (src/gpu/executor/HybridSeparatePatternMatch.cuh:91-95). */
+int sme_count_buckets(sme_ctx* ctx, const sme_graph* g, const sme_plan* p, const uint32_t* bounds,
+                      uint64_t nbuckets, uint64_t* counts, uint64_t* out_total,
+                      uint64_t* level_counts, sme_stats* stats);
+
 /* ---- kernel profile (HIP events around every launch) ------------------ */
 int sme_profile_enable(sme_ctx* ctx, int on);
 int sme_profile_reset(sme_ctx* ctx);

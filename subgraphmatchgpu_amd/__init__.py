@@ -100,6 +100,8 @@ def library() -> ctypes.CDLL:
         "sme_free_host": ([vp], None),
         "sme_calibrate_copy": ([vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
         "sme_set_hub_bitmaps": ([vp, ctypes.c_uint64, ctypes.c_uint64], ctypes.c_int),
+        "sme_count_buckets": ([vp, vp, vp, vp, ctypes.c_uint64, vp, u64p, u64p, ctypes.POINTER(_Stats)],
+                              ctypes.c_int),
         "sme_graph_hubs": ([vp, u64p], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
@@ -264,6 +266,22 @@ class Device:
                              ctypes.byref(st)), self._h)
         return CountResult(int(cnt.value), [int(lv[i]) for i in range(plan.levels)], int(st.batches),
                            int(st.d2h_syncs), int(st.peak_pool_bytes), float(st.elapsed_ms))
+
+    def count_buckets(self, dg: "DeviceGraph", plan: Plan, bounds) -> tuple:
+        """One pass over [bounds[0], bounds[-1]); returns (per-bucket counts
+        as uint64 array, CountResult of the whole pass)."""
+        lib = library()
+        b = np.ascontiguousarray(bounds, dtype=np.uint32)
+        nb = b.size - 1
+        out = np.zeros(max(nb, 1), dtype=np.uint64)
+        cnt = ctypes.c_uint64()
+        lv = (ctypes.c_uint64 * MAX_LEVELS)()
+        st = _Stats()
+        _check(lib.sme_count_buckets(self._h, dg._h, plan._h, b.ctypes.data, nb, out.ctypes.data,
+                                     ctypes.byref(cnt), lv, ctypes.byref(st)), self._h)
+        res = CountResult(int(cnt.value), [int(lv[i]) for i in range(plan.levels)], int(st.batches),
+                          int(st.d2h_syncs), int(st.peak_pool_bytes), float(st.elapsed_ms))
+        return out[:nb], res
 
     def profile(self, on: bool = True):
         _check(library().sme_profile_enable(self._h, int(on)), self._h)

@@ -180,6 +180,25 @@ int sme_count(sme_ctx* ctx, const sme_graph* g, const sme_plan* p, uint32_t v_be
   return SME_OK;
 }
 
+int sme_count_buckets(sme_ctx* ctx, const sme_graph* g, const sme_plan* p, const uint32_t* bounds,
+                      uint64_t nbuckets, uint64_t* counts, uint64_t* out_total, uint64_t* level_counts,
+                      sme_stats* stats) {
+  if (!ctx || !g || !p || !bounds || !counts || !out_total)
+    return Report(ctx, SME_ERR_INVALID, "null argument");
+  std::string err;
+  sme::RunStats rs;
+  const sme::Status s = sme::RunPlanBuckets(&ctx->impl, &g->impl, &p->plan, bounds, nbuckets, counts,
+                                            out_total, level_counts, &rs, &err);
+  if (s != sme::kOk) return Report(ctx, s, err);
+  if (stats) {
+    stats->batches = rs.batches;
+    stats->d2h_syncs = rs.d2h_syncs;
+    stats->peak_pool_bytes = rs.peak_pool_bytes;
+    stats->elapsed_ms = rs.elapsed_ms;
+  }
+  return SME_OK;
+}
+
 int sme_set_hub_bitmaps(sme_ctx* ctx, uint64_t min_degree, uint64_t max_bytes) {
   if (!ctx) return SME_ERR_INVALID;
   ctx->impl.hub_min_degree = min_degree;

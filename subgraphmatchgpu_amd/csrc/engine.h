@@ -114,4 +114,11 @@ Status RunPlan(Context* ctx, const DevGraph* g, const EnginePlan* plan, uint32_t
                uint32_t v_end, uint64_t* count, uint64_t* level_counts, RunStats* stats,
                std::string* err);
 
+// Same walk over [bounds[0], bounds[nbuckets]); additionally counts[k]
+// receives the embeddings whose first matched vertex lies in
+// [bounds[k], bounds[k+1]).  bounds must be non-decreasing.
+Status RunPlanBuckets(Context* ctx, const DevGraph* g, const EnginePlan* plan,
+                      const uint32_t* bounds, uint64_t nbuckets, uint64_t* counts, uint64_t* count,
+                      uint64_t* level_counts, RunStats* stats, std::string* err);
+
 }  // namespace sme

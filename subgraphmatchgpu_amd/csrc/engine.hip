@@ -250,6 +250,14 @@ class Runner {
   Runner(Context* ctx, const DevGraph* g, const EnginePlan* plan)
       : ctx_(ctx), g_(g), plan_(plan), st_(ctx->stream()), pool_(&ctx->pool()) {}
 
+  // Per-bucket counts (device arrays owned by the caller).
+  void SetBuckets(const uint32_t* dev_bounds, uint64_t nbuckets, unsigned long long* dev_counts) {
+    bk_.bounds = dev_bounds;
+    bk_.nbuckets = nbuckets;
+    bk_.counts = dev_counts;
+    bk_.start_v = static_cast<uint8_t>(plan_->ops[0].u);
+  }
+
   void Run(uint32_t v_begin, uint32_t v_end, uint64_t* count, uint64_t* levels, RunStats* stats) {
     Analyze();
     Buf<unsigned long long> acc(pool_, 1 + plan_->nops);
@@ -286,6 +294,11 @@ class Runner {
       for (int i = 0; i < op.nc; ++i) ref |= 1u << op.cond[i].v;
       if (op.kind == kCompute || op.kind == kComputePathCount || op.kind == kComputeCount)
         for (int i = 0; i < op.nb; ++i) ref |= 1u << op.bconn[i];
+      // per-bucket counting reads the first matched vertex at every level
+      if (bk_.bounds) {
+        ref |= 1u << bk_.start_v;
+        live_after_[L] |= 1u << bk_.start_v;
+      }
       need_after |= ref;
     }
     std::memset(compute_bconn_, 0, sizeof(compute_bconn_));
@@ -635,8 +648,24 @@ class Runner {
     if (nlists == 1 && nne == 0 && !pair) {
       // the candidate set is a window of one adjacency list: no copy
       if (op.kind == kComputeCount) {
-        host_level_[L] += W;
-        host_count_ += W;
+        if (!bk_.bounds) {
+          host_level_[L] += W;
+          host_count_ += W;
+          return;
+        }
+        // per-bucket: every path adds its window size to its bucket
+        CountPathArgs ca{};
+        ca.npaths = P;
+        ca.mode = 1;
+        ca.ncun = 1;
+        ca.cun[0] = static_cast<uint8_t>(u);
+        ca.cand[u] = CandRef{g_->cols, pstart.get(), size.get(), nullptr};
+        ca.acc = acc_;
+        ca.acc_level = acc_ + 1 + L;
+        ca.cs = f.cs;
+        ca.bk = bk_;
+        Timed t(ctx_, st_, kSlotCount, P * 16.0, P);
+        LaunchCountPaths(ca, st_);
         return;
       }
       Frame f2 = f;
@@ -693,6 +722,7 @@ class Runner {
         }
         it.acc = acc_;
         it.acc_level = acc_ + 1 + L;
+        it.bk = bk_;
         Timed t(ctx_, st_, kSlotIntersect, 4.0 * nlists * items + seg_bytes * nseg + (pair ? 20.0 * nseg : 0), items);
         LaunchVerify(it, pair ? kModeSumPair : kModeSum, st_);
         l = r;
@@ -858,6 +888,17 @@ class Runner {
         f2.reuse.idx = idx[nidx]->get();
         f2.has_reuse = true;
       }
+      const bool gpsm = plan_->algo == kAlgoGpsm || plan_->algo == kAlgoGpsmReuse;
+      if (gpsm && bk_.bounds && L + 1 == plan_->nops && C) {
+        // GPSM counts the last level's partial instances: one per path
+        CountPathArgs ca{};
+        ca.npaths = C;
+        ca.mode = 2;
+        ca.cs = f2.cs;
+        ca.bk = bk_;
+        Timed t(ctx_, st_, kSlotCount, C * 8.0, C);
+        LaunchCountPaths(ca, st_);
+      }
       Next(L, f2);
       l = r;
       il = ir;
@@ -960,6 +1001,8 @@ class Runner {
       for (int i = 0; i < op.ncun; ++i) ca.cand[op.cun[i]] = f.cand[op.cun[i]];
       ca.acc = acc_;
       ca.acc_level = acc_ + 1 + L;
+      ca.cs = f.cs;
+      ca.bk = bk_;
       Timed t(ctx_, st_, kSlotCount, P * 20.0 * op.ncun, P);
       LaunchCountPaths(ca, st_);
       return;
@@ -996,6 +1039,7 @@ class Runner {
       it.pair_op = op.pair_op;
       it.acc = acc_;
       it.acc_level = acc_ + 1 + L;
+      it.bk = bk_;
       Timed t(ctx_, st_, kSlotCount, 8.0 * items + 40.0 * nseg, items);
       LaunchVerify(it, kModeSumTwo, st_);
       l = r;
@@ -1014,6 +1058,7 @@ class Runner {
   uint64_t host_count_ = 0;
   uint32_t live_after_[kMaxOps];
   uint32_t compute_bconn_[kQ];
+  BucketRef bk_{};
   int alias_of_[kMaxOps];        // earlier level computing the same set, or -1
   int reuse_w_[kMaxOps];         // vertex whose candidate set level L extends
   bool keep_reuse_[kMaxOps];     // MATERIALIZE at L keeps the parent segment
@@ -1051,6 +1096,61 @@ Status RunPlan(Context* ctx, const DevGraph* g, const EnginePlan* plan, uint32_t
     if (stats) *stats = st;
   } catch (const Error& e) {
     // drain the stream so pool memory handed back is no longer in use
+    (void)hipStreamSynchronize(ctx->stream());
+    for (auto& m : ctx->marks) {
+      ctx->spare_events.push_back(m.start);
+      ctx->spare_events.push_back(m.stop);
+    }
+    ctx->marks.clear();
+    *err = e.msg;
+    return e.status;
+  }
+  return kOk;
+}
+
+Status RunPlanBuckets(Context* ctx, const DevGraph* g, const EnginePlan* plan, const uint32_t* bounds,
+                      uint64_t nbuckets, uint64_t* counts, uint64_t* count, uint64_t* level_counts,
+                      RunStats* stats, std::string* err) {
+  if (!ctx || !g || !plan || !count || !bounds || !counts || nbuckets == 0) {
+    *err = "null argument";
+    return kErrInvalid;
+  }
+  if (g->device != ctx->device()) {
+    *err = "graph lives on another device";
+    return kErrInvalid;
+  }
+  for (uint64_t k = 0; k < nbuckets; ++k) {
+    if (bounds[k] > bounds[k + 1]) {
+      *err = "bucket bounds must be non-decreasing";
+      return kErrInvalid;
+    }
+  }
+  if (bounds[nbuckets] > g->n) {
+    *err = "bucket bounds exceed the vertex count";
+    return kErrInvalid;
+  }
+  const auto t0 = std::chrono::steady_clock::now();
+  try {
+    SME_HIP(hipSetDevice(ctx->device()));
+    ctx->pool().ResetHighWater();
+    Buf<uint32_t> dbounds(&ctx->pool(), nbuckets + 1);
+    Buf<unsigned long long> dcounts(&ctx->pool(), nbuckets);
+    SME_HIP(hipMemcpyAsync(dbounds.get(), bounds, (nbuckets + 1) * sizeof(uint32_t), hipMemcpyHostToDevice,
+                           ctx->stream()));
+    LaunchFill64(reinterpret_cast<uint64_t*>(dcounts.get()), nbuckets, 0, ctx->stream());
+    Runner r(ctx, g, plan);
+    r.SetBuckets(dbounds.get(), nbuckets, dcounts.get());
+    RunStats st;
+    r.Run(bounds[0], bounds[nbuckets], count, level_counts, &st);
+    SME_HIP(hipMemcpyAsync(counts, dcounts.get(), nbuckets * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                           ctx->stream()));
+    SME_HIP(hipStreamSynchronize(ctx->stream()));
+    if (ctx->profiling) ctx->ResolveMarks();
+    st.peak_pool_bytes = ctx->pool().HighWater();
+    st.elapsed_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (stats) *stats = st;
+  } catch (const Error& e) {
     (void)hipStreamSynchronize(ctx->stream());
     for (auto& m : ctx->marks) {
       ctx->spare_events.push_back(m.start);

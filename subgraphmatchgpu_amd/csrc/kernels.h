@@ -74,6 +74,17 @@ struct PrepListArgs {
   uint64_t* pstart;
 };
 
+// Per-bucket attribution of counts: bucket k holds the paths whose first
+// matched vertex (column `start_v`) lies in [bounds[k], bounds[k+1]).  Paths
+// are ordered by start vertex at every level, so the lanes of a wave hold
+// non-decreasing bucket ids and one segmented wave sum per run suffices.
+struct BucketRef {
+  const uint32_t* bounds;  // nullptr: no per-bucket counts
+  uint64_t nbuckets;
+  unsigned long long* counts;
+  uint8_t start_v;
+};
+
 // Item kernels.  `offs` / per-segment arrays are already offset to the
 // batch's first segment; items are numbered from offs[0].
 struct ItemArgs {
@@ -115,6 +126,7 @@ struct ItemArgs {
   uint32_t* ccount;               // per chunk popcount
   unsigned long long* acc;        // total count
   unsigned long long* acc_level;  // this level's contribution
+  BucketRef bk;                   // counting modes only
 };
 
 enum VerifyMode : int { kModeMask = 0, kModeSum = 1, kModeSumPair = 2, kModeSumTwo = 3 };
@@ -152,13 +164,15 @@ struct CompactArgs {
 // Per-path closed-form / single-set counts (COUNT).
 struct CountPathArgs {
   uint64_t npaths;
-  uint8_t mode;  // 0: direct closed form, 1: size of one set
+  uint8_t mode;  // 0: direct closed form, 1: size of one set, 2: one per path
   int query_type;
   uint8_t ncun;
   uint8_t cun[kQ];
   CandRef cand[kQ];  // indexed by query vertex
   unsigned long long* acc;
   unsigned long long* acc_level;
+  ColSet cs;         // start-vertex column for bucket attribution
+  BucketRef bk;
 };
 
 // Per-path set-up of the two-set COUNT.

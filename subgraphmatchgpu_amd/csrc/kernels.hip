@@ -64,6 +64,37 @@ __device__ __forceinline__ uint64_t SegIndex(const CandRef& c, uint64_t p) {
   return c.idx ? c.idx[p] : p;
 }
 
+constexpr u64 kNoBucket = ~0ull;
+
+// Bucket of start vertex v: largest k with bounds[k] <= v.
+__device__ __forceinline__ u64 BucketOf(const BucketRef& b, uint32_t v) {
+  uint64_t lo = 0, hi = b.nbuckets - 1;
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi + 1) >> 1;
+    if (b.bounds[mid] <= v)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+// Segmented wave sum over lanes with equal keys (keys are non-decreasing
+// across the wave); the last lane of each run adds the run's total.  Must
+// be reached by all 64 lanes.
+__device__ __forceinline__ void BucketAdd(const BucketRef& b, u64 key, u64 v) {
+  const int lane = threadIdx.x & (kWave - 1);
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    const u64 y = __shfl_up(v, o);
+    const u64 k2 = __shfl_up(key, o);
+    if (lane >= o && k2 == key) v += y;
+  }
+  const u64 next = __shfl_down(key, 1);
+  const bool run_end = lane == kWave - 1 || next != key;
+  if (run_end && v && key != kNoBucket) atomicAdd(b.counts + key, v);
+}
+
 // ThreadCountWithComputedVertex (CheckConstraintsCommon.cuh:131-155): how
 // many y in arr satisfy  elem (op) y.
 __device__ __forceinline__ uint64_t CountWithComputed(const uint32_t* arr, uint64_t n, uint32_t elem,
@@ -207,14 +238,29 @@ __device__ uint64_t DirectCount(const uint64_t* c, int q) {
 
 __global__ __launch_bounds__(kThreads) void CountPathsKernel(CountPathArgs a) {
   u64 local = 0;
-  for (uint64_t p = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; p < a.npaths;
-       p += GridStride()) {
-    uint64_t c[kQ] = {0, 0, 0, 0, 0, 0};
-    for (int i = 0; i < a.ncun; ++i) {
-      const int u = a.cun[i];
-      c[u] = a.cand[u].len[SegIndex(a.cand[u], p)];
+  const int lane = threadIdx.x & (kWave - 1);
+  // wave-uniform trip count, so per-bucket sums can use the whole wave
+  for (uint64_t base = blockIdx.x * static_cast<uint64_t>(blockDim.x) + (threadIdx.x - lane);
+       base < a.npaths; base += GridStride()) {
+    const uint64_t p = base + lane;
+    u64 val = 0;
+    if (p < a.npaths) {
+      if (a.mode == 2) {
+        val = 1;
+      } else {
+        uint64_t c[kQ] = {0, 0, 0, 0, 0, 0};
+        for (int i = 0; i < a.ncun; ++i) {
+          const int u = a.cun[i];
+          c[u] = a.cand[u].len[SegIndex(a.cand[u], p)];
+        }
+        val = a.mode == 0 ? DirectCount(c, a.query_type) : c[a.cun[0]];
+      }
     }
-    local += a.mode == 0 ? DirectCount(c, a.query_type) : c[a.cun[0]];
+    local += val;
+    if (a.bk.bounds) {
+      const u64 key = p < a.npaths ? BucketOf(a.bk, Gather(a.cs, a.bk.start_v, p)) : kNoBucket;
+      BucketAdd(a.bk, key, val);
+    }
   }
   local = WaveSum(local);
   if ((threadIdx.x & (kWave - 1)) == 0 && local) {
@@ -280,7 +326,7 @@ __global__ __launch_bounds__(kThreads) void VerifyKernel(ItemArgs a) {
     const uint64_t t = c * kWave + lane;
     bool ok = t < a.nitems;
     uint64_t s = 0;
-    uint32_t x = 0;
+    u64 val = 0;
     if (ok) {
       s = SegmentOf(a, c, t);
       const uint64_t r = a.offs[0] + t - a.offs[s];
@@ -290,24 +336,28 @@ __global__ __launch_bounds__(kThreads) void VerifyKernel(ItemArgs a) {
         const bool sw = a.swapped[s] != 0;
         const CandRef& sm = sw ? a.second : a.pair;
         const CandRef& bg = sw ? a.pair : a.second;
-        x = sm.base[a.pstart[s] + r];
+        const uint32_t x = sm.base[a.pstart[s] + r];
         const uint64_t bs = SegIndex(bg, s);
         const uint32_t* arr = bg.base + bg.beg[bs];
         const uint64_t n = bg.len[bs];
         const uint64_t pos = LowerBound(arr, 0, n, x);
         const bool eq = pos < n && arr[pos] == x;
-        uint64_t add;
         if (a.pair_op == 0)
-          add = sw ? pos : n - pos - (eq ? 1 : 0);
+          val = sw ? pos : n - pos - (eq ? 1 : 0);
         else if (a.pair_op == 1)
-          add = sw ? n - pos - (eq ? 1 : 0) : pos;
+          val = sw ? n - pos - (eq ? 1 : 0) : pos;
         else
-          add = n - (eq ? 1 : 0);
-        local += add;
-        continue;
+          val = n - (eq ? 1 : 0);
+      } else {
+        const uint32_t x = ItemSrc(a, s)[a.pstart[s] + r];
+        ok = ItemValid(a, s, x);
+        if (kMode == kModeSum) {
+          val = ok ? 1 : 0;
+        } else if (kMode == kModeSumPair && ok) {
+          const uint64_t ps = SegIndex(a.pair, s);
+          val = CountWithComputed(a.pair.base + a.pair.beg[ps], a.pair.len[ps], x, a.pair_op);
+        }
       }
-      x = ItemSrc(a, s)[a.pstart[s] + r];
-      ok = ItemValid(a, s, x);
     }
     if (kMode == kModeMask) {
       const u64 m = __ballot(ok);
@@ -315,12 +365,11 @@ __global__ __launch_bounds__(kThreads) void VerifyKernel(ItemArgs a) {
         a.mask[c] = m;
         a.ccount[c] = static_cast<uint32_t>(__popcll(m));
       }
-    } else if (kMode == kModeSum) {
-      local += ok ? 1 : 0;
-    } else if (kMode == kModeSumPair) {
-      if (ok) {
-        const uint64_t ps = SegIndex(a.pair, s);
-        local += CountWithComputed(a.pair.base + a.pair.beg[ps], a.pair.len[ps], x, a.pair_op);
+    } else {
+      local += val;
+      if (a.bk.bounds) {
+        const u64 key = t < a.nitems ? BucketOf(a.bk, Gather(a.cs, a.bk.start_v, s)) : kNoBucket;
+        BucketAdd(a.bk, key, val);
       }
     }
   }

@@ -4,6 +4,7 @@
 // Output: "count=<n> levels=<l0>,<l1>,... batches=<b>"
 #include <cstdio>
 #include <cstdlib>
+#include <vector>
 
 #include "../../include/sme.h"
 
@@ -44,7 +45,19 @@ int main(int argc, char** argv) {
   }
   uint64_t count = 0, levels[SME_MAX_LEVELS] = {0};
   sme_stats st{};
-  const int rc = sme_count(ctx, g, plan, vb, static_cast<uint32_t>(ve), &count, levels, &st);
+  // BUCKETS="b0,b1,...,bk" switches to sme_count_buckets over those bounds
+  std::vector<uint32_t> bounds;
+  if (const char* env = std::getenv("BUCKETS")) {
+    for (const char* s = env; *s;) {
+      bounds.push_back(static_cast<uint32_t>(std::strtoul(s, const_cast<char**>(&s), 10)));
+      if (*s == ',') ++s;
+    }
+  }
+  std::vector<uint64_t> bcounts(bounds.empty() ? 0 : bounds.size() - 1);
+  const int rc = bounds.empty()
+                     ? sme_count(ctx, g, plan, vb, static_cast<uint32_t>(ve), &count, levels, &st)
+                     : sme_count_buckets(ctx, g, plan, bounds.data(), bcounts.size(), bcounts.data(),
+                                         &count, levels, &st);
   if (rc != SME_OK) {
     std::fprintf(stderr, "count: rc=%d %s\n", rc, sme_last_error(ctx));
     return 1;
@@ -52,7 +65,13 @@ int main(int argc, char** argv) {
   std::printf("count=%llu levels=", static_cast<unsigned long long>(count));
   for (int i = 0; i < sme_plan_levels(plan); ++i)
     std::printf(i ? ",%llu" : "%llu", static_cast<unsigned long long>(levels[i]));
-  std::printf(" batches=%llu\n", static_cast<unsigned long long>(st.batches));
+  std::printf(" batches=%llu", static_cast<unsigned long long>(st.batches));
+  if (!bounds.empty()) {
+    std::printf(" buckets=");
+    for (size_t k = 0; k < bcounts.size(); ++k)
+      std::printf(k ? ",%llu" : "%llu", static_cast<unsigned long long>(bcounts[k]));
+  }
+  std::printf("\n");
   sme_graph_destroy(g);
   sme_plan_destroy(plan);
   sme_close(ctx);

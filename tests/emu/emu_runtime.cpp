@@ -186,6 +186,9 @@ void Launch(dim3 grid, dim3 block, const std::function<void()>& body) {
           } else if (kind == kShflXor) {
             const unsigned src = i ^ l.arg;
             l.result = (src < 64 && src < hi - lo) ? vals[src] : vals[i];
+          } else if (kind == kShflDown) {
+            const unsigned src = i + l.arg;
+            l.result = src < hi - lo ? vals[src] : vals[i];
           } else {
             l.result = i >= l.arg ? vals[i - l.arg] : vals[i];
           }

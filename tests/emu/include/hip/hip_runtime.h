@@ -69,7 +69,7 @@ extern dim3 g_block_idx, g_block_dim, g_grid_dim;
 void Launch(dim3 grid, dim3 block, const std::function<void()>& body);
 unsigned long long WaveExchange(int kind, unsigned long long v, unsigned arg);
 void BlockBarrier();
-enum { kBallot = 1, kShflXor = 2, kShflUp = 3 };
+enum { kBallot = 1, kShflXor = 2, kShflUp = 3, kShflDown = 4 };
 template <typename T>
 inline T FromBits(unsigned long long b) {
   T t;
@@ -100,6 +100,11 @@ template <typename T>
 inline T __shfl_up(T v, unsigned delta, int width = 64) {
   (void)width;
   return emu::FromBits<T>(emu::WaveExchange(emu::kShflUp, emu::ToBits(v), delta));
+}
+template <typename T>
+inline T __shfl_down(T v, unsigned delta, int width = 64) {
+  (void)width;
+  return emu::FromBits<T>(emu::WaveExchange(emu::kShflDown, emu::ToBits(v), delta));
 }
 inline unsigned int __popcll(unsigned long long x) { return static_cast<unsigned>(__builtin_popcountll(x)); }
 inline unsigned long long __lanemask_lt() {

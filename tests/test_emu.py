@@ -97,6 +97,23 @@ def test_emu_hub_bitmaps(emu, graphs):
                 (want[0], want[1]), (q, thr)
 
 
+def test_emu_buckets(emu, graphs):
+    """Per-start-vertex-bucket counts of one pass equal separate range counts."""
+    g, p = graphs["rmat7"]
+    bounds = [0, 3, 3, 17, 40, 41, 90, g.n]
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0", BUCKETS=",".join(map(str, bounds)))
+    for q in (0, 1, 2, 4, 7):
+        for algo in (6, 4):
+            r = subprocess.run([emu, p, str(algo), str(q), "3", "1", "0", str(g.n), str(1 << 30)],
+                               capture_output=True, text=True, env=env, timeout=600)
+            assert r.returncode == 0, r.stderr[-2000:]
+            got = [int(x) for x in r.stdout.split("buckets=")[1].split(",")]
+            count = oracle.light_count if algo == 6 else oracle.gpsm_count
+            want = [count(g.row_ptrs, g.cols, q, v_begin=a, v_end=b)[0]
+                    for a, b in zip(bounds[:-1], bounds[1:])]
+            assert got == want, (q, algo)
+
+
 def test_emu_clique_without_ordering(emu, graphs):
     g, p = graphs["k7"]
     for q in (0, 3, 7):

@@ -154,6 +154,20 @@ def test_hub_bitmaps(fixture_graphs):
     d.close()
 
 
+def test_buckets_match_ranges(dev, small):
+    """One pass with per-start-vertex buckets equals one call per range."""
+    g, dg = small["rmat10"]
+    bounds = [0, 1, 7, 7, 300, 301, 700, g.n]
+    for q in (0, 1, 2, 3, 4, 7):
+        for algo in (sme.GPU_LIGHT, sme.GPU_GPSM):
+            say(f"case buckets q={q} algo={algo}")
+            plan = sme.Plan(q, algo)
+            got, whole = dev.count_buckets(dg, plan, bounds)
+            want = [dev.count(dg, plan, a, b).count for a, b in zip(bounds[:-1], bounds[1:])]
+            assert got.tolist() == want
+            assert whole.count == sum(want)
+
+
 def test_edge_graphs(dev):
     empty = sme.from_adjacency([[] for _ in range(10)])
     dg = dev.upload(empty)
