@@ -137,6 +137,49 @@ int sme_count_buckets(sme_ctx* ctx, const sme_graph* g, const sme_plan* p, const
                       uint64_t nbuckets, uint64_t* counts, uint64_t* out_total,
                       uint64_t* level_counts, sme_stats* stats);
 
+/* ---- schedule tuning ---------------------------------------------------
+ * No counterpart in the reference, whose launch shapes are compile-time
+ * constants (gpu/utils/ops/GPUIntersection.cuh, LIGHTCommon.cuh).
+ *   lane_path_max  counting levels (COMPUTE_COUNT): a path whose pivot list
+ *                  window holds at most this many candidates is verified by
+ *                  one lane of the set-up kernel; longer paths go through the
+ *                  64-candidates-per-wavefront schedule.  0: every path uses
+ *                  the wavefront schedule (the default).
+ *   item_grid      workgroup cap of the candidate kernels (0: 16384); every
+ *                  wavefront loops over 64-candidate chunks, so a cap near
+ *                  the resident capacity makes the launch persistent.
+ *   batch_bytes    cap on the device bytes one batch of a level may use
+ *                  (0: the level's share of the memory budget).
+ * Settings apply to every later call on the context; results never depend
+ * on them. */
+typedef struct {
+  uint64_t lane_path_max;
+  uint32_t item_grid;
+  uint32_t reserved;
+  uint64_t batch_bytes;
+} sme_tuning;
+int sme_set_tuning(sme_ctx* ctx, const sme_tuning* t);
+int sme_get_tuning(const sme_ctx* ctx, sme_tuning* t);
+
+typedef struct {
+  sme_tuning tuning;
+  double ms; /* summed time of the sample runs */
+} sme_tune_trial;
+
+/* Autotuner: times the plans on a sample of the start vertices for each
+ * candidate setting (coordinate descent over lane_path_max, item_grid and
+ * batch_bytes, starting from the context's current setting), installs the
+ * fastest on the context and returns it.  With bounds (nbuckets >= 1) the
+ * sample is 8 evenly spaced blocks of consecutive buckets covering about
+ * sample_fraction of the buckets, run through sme_count_buckets; without
+ * (bounds == NULL) it is 8 evenly spaced start-vertex ranges covering
+ * sample_fraction of [0, n).  trials (nullable, cap entries) receives every
+ * measured setting, *ntrials their number. */
+int sme_autotune(sme_ctx* ctx, const sme_graph* g, const sme_plan* const* plans, int nplans,
+                 const uint32_t* bounds, uint64_t nbuckets, double sample_fraction,
+                 sme_tuning* best, double* best_ms, sme_tune_trial* trials, int cap,
+                 int* ntrials);
+
 /* ---- kernel profile (HIP events around every launch) ------------------ */
 int sme_profile_enable(sme_ctx* ctx, int on);
 int sme_profile_reset(sme_ctx* ctx);
@@ -162,6 +205,15 @@ int sme_gen_rmat(int scale, int edge_factor, double a, double b, double c, uint6
                  uint64_t* n, uint64_t* nnz, uint64_t** row_ptrs, uint32_t** cols);
 int sme_gen_er(uint64_t n_vertices, uint64_t n_edges, uint64_t seed, uint64_t* n, uint64_t* nnz,
                uint64_t** row_ptrs, uint32_t** cols);
+/* `ngraphs` seeded G(n, m) graphs packed back to back into one CSR (a
+ * disjoint union): member k has n_k uniform in [n_min, n_max] vertices and
+ * average degree uniform in [deg_min, deg_max); its vertices are
+ * [graph_offs[k], graph_offs[k+1]) (ngraphs + 1 offsets, released with
+ * sme_free_host).  Counting with sme_count_buckets over graph_offs gives
+ * every member's count in one pass. */
+int sme_gen_er_batch(uint64_t ngraphs, uint64_t n_min, uint64_t n_max, double deg_min,
+                     double deg_max, uint64_t seed, uint64_t* n, uint64_t* nnz,
+                     uint64_t** row_ptrs, uint32_t** cols, uint64_t** graph_offs);
 void sme_free_host(void* p);
 
 #ifdef __cplusplus

@@ -52,6 +52,19 @@ class _Stats(ctypes.Structure):
                 ("peak_pool_bytes", ctypes.c_uint64), ("elapsed_ms", ctypes.c_double)]
 
 
+class _Tuning(ctypes.Structure):
+    _fields_ = [("lane_path_max", ctypes.c_uint64), ("item_grid", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32), ("batch_bytes", ctypes.c_uint64)]
+
+    def as_dict(self) -> dict:
+        return {"lane_path_max": int(self.lane_path_max), "item_grid": int(self.item_grid),
+                "batch_bytes": int(self.batch_bytes)}
+
+
+class _Trial(ctypes.Structure):
+    _fields_ = [("tuning", _Tuning), ("ms", ctypes.c_double)]
+
+
 _lib = None
 
 
@@ -97,12 +110,21 @@ def library() -> ctypes.CDLL:
                           ctypes.POINTER(u32p)], ctypes.c_int),
         "sme_gen_er": ([ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, u64p, u64p,
                         ctypes.POINTER(u64p), ctypes.POINTER(u32p)], ctypes.c_int),
+        "sme_gen_er_batch": ([ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_double,
+                              ctypes.c_double, ctypes.c_uint64, u64p, u64p, ctypes.POINTER(u64p),
+                              ctypes.POINTER(u32p), ctypes.POINTER(u64p)], ctypes.c_int),
         "sme_free_host": ([vp], None),
         "sme_calibrate_copy": ([vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
         "sme_set_hub_bitmaps": ([vp, ctypes.c_uint64, ctypes.c_uint64], ctypes.c_int),
         "sme_count_buckets": ([vp, vp, vp, vp, ctypes.c_uint64, vp, u64p, u64p, ctypes.POINTER(_Stats)],
                               ctypes.c_int),
         "sme_graph_hubs": ([vp, u64p], ctypes.c_int),
+        "sme_set_tuning": ([vp, ctypes.POINTER(_Tuning)], ctypes.c_int),
+        "sme_get_tuning": ([vp, ctypes.POINTER(_Tuning)], ctypes.c_int),
+        "sme_autotune": ([vp, vp, ctypes.POINTER(vp), ctypes.c_int, vp, ctypes.c_uint64,
+                          ctypes.c_double, ctypes.POINTER(_Tuning), ctypes.POINTER(ctypes.c_double),
+                          ctypes.POINTER(_Trial), ctypes.c_int, ctypes.POINTER(ctypes.c_int)],
+                         ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -169,6 +191,23 @@ def gen_er(n_vertices: int, n_edges: int, seed=1) -> Graph:
     _check(library().sme_gen_er(n_vertices, n_edges, seed, ctypes.byref(n), ctypes.byref(nnz),
                                 ctypes.byref(rp), ctypes.byref(cols)))
     return _take_host(n, nnz, rp, cols)
+
+
+def gen_er_batch(ngraphs: int, n_range=(64, 8192), degree_range=(4.0, 48.0), seed=1):
+    """`ngraphs` seeded G(n, m) graphs packed back to back (a disjoint union).
+
+    Returns (graph, graph_offs): member k owns vertices
+    [graph_offs[k], graph_offs[k+1]); n_k is uniform in n_range and the
+    average degree uniform in degree_range."""
+    n, nnz, rp, cols = _out_args()
+    offs = ctypes.POINTER(ctypes.c_uint64)()
+    _check(library().sme_gen_er_batch(ngraphs, n_range[0], n_range[1], float(degree_range[0]),
+                                      float(degree_range[1]), seed, ctypes.byref(n),
+                                      ctypes.byref(nnz), ctypes.byref(rp), ctypes.byref(cols),
+                                      ctypes.byref(offs)))
+    go = np.ctypeslib.as_array(offs, shape=(ngraphs + 1,)).copy()
+    library().sme_free_host(ctypes.cast(offs, ctypes.c_void_p))
+    return _take_host(n, nnz, rp, cols), go
 
 
 def read_bin(path: str) -> Graph:
@@ -305,6 +344,38 @@ class Device:
         """Adjacency bitmaps for vertices of degree >= min_degree in graphs
         uploaded afterwards (0 disables)."""
         _check(library().sme_set_hub_bitmaps(self._h, min_degree, max_bytes), self._h)
+
+    def set_tuning(self, lane_path_max: int = 0, item_grid: int = 0, batch_bytes: int = 0):
+        """Schedule tunables for later calls (results never depend on them):
+        lane_path_max -- counting-level paths with at most this many pivot
+        candidates are verified by one lane each; item_grid -- workgroup cap
+        of the candidate kernels (0: default); batch_bytes -- cap on one
+        batch's device bytes (0: the level's share of the budget)."""
+        t = _Tuning(lane_path_max, item_grid, 0, batch_bytes)
+        _check(library().sme_set_tuning(self._h, ctypes.byref(t)), self._h)
+
+    def get_tuning(self) -> dict:
+        t = _Tuning()
+        _check(library().sme_get_tuning(self._h, ctypes.byref(t)), self._h)
+        return t.as_dict()
+
+    def autotune(self, dg: "DeviceGraph", plans, bounds=None, sample_fraction: float = 0.02):
+        """Times `plans` on a sample for each candidate setting, installs the
+        fastest and returns (tuning, sample_ms, trials)."""
+        plans = list(plans)
+        arr = (ctypes.c_void_p * len(plans))(*[p._h for p in plans])
+        if bounds is not None:
+            b = np.ascontiguousarray(bounds, dtype=np.uint32)
+            bp, nb = b.ctypes.data, b.size - 1
+        else:
+            b, bp, nb = None, None, 0
+        best, ms, n = _Tuning(), ctypes.c_double(), ctypes.c_int()
+        trials = (_Trial * 64)()
+        _check(library().sme_autotune(self._h, dg._h, arr, len(plans), bp, nb, sample_fraction,
+                                      ctypes.byref(best), ctypes.byref(ms), trials, 64,
+                                      ctypes.byref(n)), self._h)
+        log = [dict(trials[i].tuning.as_dict(), ms=float(trials[i].ms)) for i in range(min(n.value, 64))]
+        return best.as_dict(), float(ms.value), log
 
     def calibrate_copy(self, nbytes: int) -> float:
         """One device-to-device copy of nbytes inside the pool; returns ms."""

@@ -1,10 +1,13 @@
 // capi.cpp -- the extern "C" boundary declared in include/sme.h.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
+#include <utility>
+#include <vector>
 
 #include "../../include/sme.h"
 #include "engine.h"
@@ -199,6 +202,132 @@ int sme_count_buckets(sme_ctx* ctx, const sme_graph* g, const sme_plan* p, const
   return SME_OK;
 }
 
+int sme_set_tuning(sme_ctx* ctx, const sme_tuning* t) {
+  if (!ctx || !t) return Report(ctx, SME_ERR_INVALID, "null argument");
+  ctx->impl.tuning.lane_path_max = t->lane_path_max;
+  ctx->impl.tuning.item_grid = t->item_grid;
+  ctx->impl.tuning.batch_bytes = t->batch_bytes;
+  return SME_OK;
+}
+
+int sme_get_tuning(const sme_ctx* ctx, sme_tuning* t) {
+  if (!ctx || !t) return SME_ERR_INVALID;
+  *t = sme_tuning{};
+  t->lane_path_max = ctx->impl.tuning.lane_path_max;
+  t->item_grid = ctx->impl.tuning.item_grid;
+  t->batch_bytes = ctx->impl.tuning.batch_bytes;
+  return SME_OK;
+}
+
+int sme_autotune(sme_ctx* ctx, const sme_graph* g, const sme_plan* const* plans, int nplans,
+                 const uint32_t* bounds, uint64_t nbuckets, double sample_fraction,
+                 sme_tuning* best, double* best_ms, sme_tune_trial* trials, int cap,
+                 int* ntrials) {
+  if (!ctx || !g || !plans || nplans <= 0 || !best)
+    return Report(ctx, SME_ERR_INVALID, "null argument");
+  if (!(sample_fraction > 0.0) || sample_fraction > 1.0)
+    return Report(ctx, SME_ERR_INVALID, "sample_fraction must be in (0, 1]");
+  for (int i = 0; i < nplans; ++i)
+    if (!plans[i]) return Report(ctx, SME_ERR_INVALID, "null plan");
+  // the sample: 8 evenly spaced blocks of buckets or of start vertices
+  constexpr int kBlocks = 8;
+  std::vector<std::pair<uint64_t, uint64_t>> blocks;  // [lo, hi) of buckets or vertices
+  const uint64_t units = bounds ? nbuckets : g->impl.n;
+  if (units == 0) return Report(ctx, SME_ERR_INVALID, "empty sample space");
+  const uint64_t per = std::max<uint64_t>(
+      1, static_cast<uint64_t>(sample_fraction * static_cast<double>(units) / kBlocks + 0.5));
+  for (int b = 0; b < kBlocks; ++b) {
+    const uint64_t lo = units * b / kBlocks;
+    const uint64_t hi = std::min(units, lo + per);
+    if (hi > lo && (blocks.empty() || lo >= blocks.back().second)) blocks.push_back({lo, hi});
+  }
+  sme::Context& c = ctx->impl;
+  std::string err;
+  std::vector<uint64_t> counts;
+  auto measure = [&](const sme::Context::Tuning& t, double* ms) -> int {
+    c.tuning = t;
+    double total = 0;
+    for (int i = 0; i < nplans; ++i) {
+      for (const auto& blk : blocks) {
+        uint64_t cnt = 0;
+        sme::RunStats rs;
+        sme::Status s;
+        if (bounds) {
+          counts.assign(blk.second - blk.first, 0);
+          s = sme::RunPlanBuckets(&c, &g->impl, &plans[i]->plan, bounds + blk.first,
+                                  blk.second - blk.first, counts.data(), &cnt, nullptr, &rs, &err);
+        } else {
+          s = sme::RunPlan(&c, &g->impl, &plans[i]->plan, static_cast<uint32_t>(blk.first),
+                           static_cast<uint32_t>(blk.second), &cnt, nullptr, &rs, &err);
+        }
+        if (s != sme::kOk) return Report(ctx, s, err);
+        total += rs.elapsed_ms;
+      }
+    }
+    *ms = total;
+    return SME_OK;
+  };
+  const sme::Context::Tuning start = c.tuning;
+  int n = 0;
+  auto record = [&](const sme::Context::Tuning& t, double ms) {
+    if (trials && n < cap) {
+      trials[n].tuning = sme_tuning{t.lane_path_max, t.item_grid, 0, t.batch_bytes};
+      trials[n].ms = ms;
+    }
+    ++n;
+  };
+  // warm the caches and the pool once, then coordinate descent
+  double ms = 0;
+  int rc = measure(start, &ms);
+  if (rc != SME_OK) {
+    c.tuning = start;
+    return rc;
+  }
+  sme::Context::Tuning cur = start;
+  double cur_ms = 0;
+  rc = measure(cur, &cur_ms);
+  if (rc != SME_OK) {
+    c.tuning = start;
+    return rc;
+  }
+  record(cur, cur_ms);
+  const uint64_t lane_opts[] = {0, 8, 32, 128, 512};
+  const uint32_t grid_opts[] = {0, 8192, 4096, 2048, 1024};
+  const uint64_t batch_opts[] = {0, 4ull << 30, 1ull << 30};
+  for (int dim = 0; dim < 3; ++dim) {
+    const int nopt = dim == 0 ? 5 : dim == 1 ? 5 : 3;
+    sme::Context::Tuning dim_best = cur;
+    double dim_ms = cur_ms;
+    for (int k = 0; k < nopt; ++k) {
+      sme::Context::Tuning t = cur;
+      if (dim == 0) t.lane_path_max = lane_opts[k];
+      if (dim == 1) t.item_grid = grid_opts[k];
+      if (dim == 2) t.batch_bytes = batch_opts[k];
+      if (t.lane_path_max == cur.lane_path_max && t.item_grid == cur.item_grid &&
+          t.batch_bytes == cur.batch_bytes)
+        continue;
+      double t_ms = 0;
+      rc = measure(t, &t_ms);
+      if (rc != SME_OK) {
+        c.tuning = start;
+        return rc;
+      }
+      record(t, t_ms);
+      if (t_ms < dim_ms) {
+        dim_best = t;
+        dim_ms = t_ms;
+      }
+    }
+    cur = dim_best;
+    cur_ms = dim_ms;
+  }
+  c.tuning = cur;
+  *best = sme_tuning{cur.lane_path_max, cur.item_grid, 0, cur.batch_bytes};
+  if (best_ms) *best_ms = cur_ms;
+  if (ntrials) *ntrials = n;
+  return SME_OK;
+}
+
 int sme_set_hub_bitmaps(sme_ctx* ctx, uint64_t min_degree, uint64_t max_bytes) {
   if (!ctx) return SME_ERR_INVALID;
   ctx->impl.hub_min_degree = min_degree;
@@ -251,7 +380,7 @@ int sme_calibrate_copy(sme_ctx* ctx, uint64_t bytes, double* ms) {
     return Report(ctx, SME_ERR_OOM, "pool too small for the copy");
   }
   hipStream_t st = ctx->impl.stream();
-  hipEvent_t e0, e1;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
   int rc = SME_OK;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess ||
       hipMemsetAsync(a, 1, bytes, st) != hipSuccess || hipEventRecord(e0, st) != hipSuccess ||
@@ -319,6 +448,28 @@ int sme_gen_er(uint64_t n_vertices, uint64_t n_edges, uint64_t seed, uint64_t* n
   sme::HostGraph g;
   sme::GenerateErdosRenyi(n_vertices, n_edges, seed, &g);
   return HandOut(g, n, nnz, row_ptrs, cols);
+}
+
+int sme_gen_er_batch(uint64_t ngraphs, uint64_t n_min, uint64_t n_max, double deg_min,
+                     double deg_max, uint64_t seed, uint64_t* n, uint64_t* nnz,
+                     uint64_t** row_ptrs, uint32_t** cols, uint64_t** graph_offs) {
+  if (!n || !nnz || !row_ptrs || !cols || !graph_offs)
+    return Report(nullptr, SME_ERR_INVALID, "null argument");
+  if (ngraphs == 0 || n_min < 2 || n_max < n_min || !(deg_min >= 0.0) || deg_max < deg_min)
+    return Report(nullptr, SME_ERR_INVALID, "bad member ranges");
+  if (ngraphs > (0xffffffffull - 1) / n_max)
+    return Report(nullptr, SME_ERR_INVALID, "packed vertex ids exceed 32 bits");
+  sme::HostGraph g;
+  std::vector<uint64_t> offs;
+  sme::GenerateErdosRenyiBatch(ngraphs, n_min, n_max, deg_min, deg_max, seed, &g, &offs);
+  *graph_offs = CopyOut(offs);
+  if (!*graph_offs) return Report(nullptr, SME_ERR_OOM, "host allocation failed");
+  const int rc = HandOut(g, n, nnz, row_ptrs, cols);
+  if (rc != SME_OK) {
+    std::free(*graph_offs);
+    *graph_offs = nullptr;
+  }
+  return rc;
 }
 
 void sme_free_host(void* p) { std::free(p); }

@@ -29,6 +29,7 @@ enum KernelSlot : int {
   kSlotScan,
   kSlotPartition,
   kSlotIntersect,   // item verify kernels of COMPUTE / CPC / COMPUTE_COUNT
+  kSlotLane,        // COMPUTE_COUNT set-up with lane-per-path counting
   kSlotFilter,      // item verify kernels of MATERIALIZE / FILTER_COMPUTE
   kSlotCount,       // COUNT kernels
   kSlotCompact,     // candidate compaction + segment windows
@@ -66,6 +67,20 @@ class Context {
   // (0 disables)
   uint64_t hub_min_degree = 0;
   uint64_t hub_max_bytes = 0;
+
+  // Schedule tunables (chosen by hand or by the autotuner):
+  //   lane_path_max  COMPUTE_COUNT paths with at most this many pivot items
+  //                  are counted by one lane in the set-up kernel; longer
+  //                  paths go through the 64-items-per-wave schedule (0: all
+  //                  paths use the item schedule)
+  //   item_grid      workgroup cap of the item kernels (0: default)
+  //   batch_bytes    cap on the device bytes of one batch (0: the level's
+  //                  share of the pool)
+  struct Tuning {
+    uint64_t lane_path_max = 0;
+    uint32_t item_grid = 0;
+    uint64_t batch_bytes = 0;
+  } tuning;
 
   // profiling
   bool profiling = false;

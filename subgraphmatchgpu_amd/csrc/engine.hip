@@ -22,8 +22,9 @@ void Fail(Status s, const std::string& msg) { throw Error{s, msg}; }
   } while (0)
 
 const char* SlotName(int slot) {
-  static const char* kNames[kNumSlots] = {"prep",  "scan",    "partition", "intersect", "filter",
-                                          "count", "compact", "expand",    "misc"};
+  static const char* kNames[kNumSlots] = {"prep",  "scan",   "partition", "intersect",
+                                          "intersect_lane", "filter", "count", "compact",
+                                          "expand", "misc"};
   return slot >= 0 && slot < kNumSlots ? kNames[slot] : "?";
 }
 
@@ -260,18 +261,30 @@ class Runner {
 
   void Run(uint32_t v_begin, uint32_t v_end, uint64_t* count, uint64_t* levels, RunStats* stats) {
     Analyze();
-    Buf<unsigned long long> acc(pool_, 1 + plan_->nops);
+    // [total, per-level counts, per-level items counted by lanes]
+    const int nacc = 1 + 2 * plan_->nops;
+    Buf<unsigned long long> acc(pool_, nacc);
     acc_ = acc.get();
-    SME_HIP(hipMemsetAsync(acc_, 0, sizeof(unsigned long long) * (1 + plan_->nops), st_));
+    SME_HIP(hipMemsetAsync(acc_, 0, sizeof(unsigned long long) * nacc, st_));
+    std::memset(lane_lists_, 0, sizeof(lane_lists_));
     std::memset(host_level_, 0, sizeof(host_level_));
     host_count_ = 0;
     if (v_end > v_begin) Level0(v_begin, v_end);
-    unsigned long long dev[1 + kMaxOps];
-    SME_HIP(hipMemcpyAsync(ctx_->pinned(), acc_, sizeof(unsigned long long) * (1 + plan_->nops),
+    unsigned long long dev[1 + 2 * kMaxOps];
+    SME_HIP(hipMemcpyAsync(ctx_->pinned(), acc_, sizeof(unsigned long long) * nacc,
                            hipMemcpyDeviceToHost, st_));
     SME_HIP(hipStreamSynchronize(st_));
-    std::memcpy(dev, ctx_->pinned(), sizeof(unsigned long long) * (1 + plan_->nops));
+    std::memcpy(dev, ctx_->pinned(), sizeof(unsigned long long) * nacc);
     stats_.d2h_syncs++;
+    if (ctx_->profiling) {
+      // lane-counted items are known only now: 4 bytes per list per item
+      SlotStats& ls = ctx_->slots[kSlotLane];
+      for (int L = 0; L < plan_->nops; ++L) {
+        const uint64_t items = dev[1 + plan_->nops + L];
+        ls.bytes += 4.0 * lane_lists_[L] * static_cast<double>(items);
+        ls.items += items;
+      }
+    }
     uint64_t total = host_count_ + dev[0];
     for (int L = 0; L < plan_->nops; ++L) levels_[L] = host_level_[L] + dev[1 + L];
     // GPSM counts the size of the last level (HybridGPSMGPUProcessor.cuh:23-33)
@@ -409,8 +422,9 @@ class Runner {
 
   uint64_t Budget(int L) const {
     const uint64_t avail = std::min<uint64_t>(pool_->Available(), pool_->LargestFree());
-    const uint64_t per = static_cast<uint64_t>(0.9 * static_cast<double>(avail) /
-                                               static_cast<double>(plan_->nops - L));
+    uint64_t per = static_cast<uint64_t>(0.9 * static_cast<double>(avail) /
+                                         static_cast<double>(plan_->nops - L));
+    if (ctx_->tuning.batch_bytes) per = std::min<uint64_t>(per, ctx_->tuning.batch_bytes);
     return std::max<uint64_t>(per, 4096);
   }
 
@@ -630,13 +644,6 @@ class Runner {
     pa.size = size.get();
     pa.pstart = pstart.get();
     pa.piv = piv.get();
-    {
-      Timed t(ctx_, st_, kSlotPrep,
-              P * (4.0 * (nlists + pa.nlt + pa.ngt) + 16.0 * nlists + 8.0 + 17.0), P);
-      LaunchPrepCompute(pa, st_);
-    }
-    Scan(size.get(), P, offs.get());
-    const uint64_t W = ReadU64(offs.get() + P);
 
     uint32_t bmask = 0;
     for (int i = 0; i < op.nb; ++i) bmask |= 1u << op.bconn[i];
@@ -644,8 +651,45 @@ class Runner {
     const uint8_t nne = RealNonEqual(op, bmask, ne_v);
     const bool pair = op.kind == kComputeCount && op.ncun == 1;
     const int u = op.u;
+    const bool zero_copy = nlists == 1 && nne == 0 && !pair;
+    const double prep_bytes = P * (4.0 * (nlists + pa.nlt + pa.ngt) + 16.0 * nlists + 8.0 + 17.0);
+    // short paths of a counting level: one lane per path in the set-up kernel
+    const bool lane_count =
+        op.kind == kComputeCount && !zero_copy && ctx_->tuning.lane_path_max > 0;
+    if (lane_count) {
+      PrepCountArgs pc{};
+      pc.pa = pa;
+      pc.it = BaseItems(f, 0, P, 0, nullptr, nullptr, 0);
+      pc.it.nb = nb;
+      std::memcpy(pc.it.bconn, lists, kQ);
+      if (reuse) {
+        pc.it.has_reuse = 1;
+        pc.it.reuse = f.reuse;
+      }
+      pc.it.nne = nne;
+      std::memcpy(pc.it.ne_v, ne_v, sizeof(pc.it.ne_v));
+      if (pair) {
+        pc.it.pair = f.cand[op.cun[0]];
+        pc.it.pair_op = op.pair_op;
+      }
+      pc.it.acc = acc_;
+      pc.it.acc_level = acc_ + 1 + L;
+      pc.it.bk = bk_;
+      pc.lane_max = ctx_->tuning.lane_path_max;
+      pc.pair = pair ? 1 : 0;
+      pc.lane_items = acc_ + 1 + plan_->nops + L;
+      lane_lists_[L] = nlists;
+      Timed t(ctx_, st_, kSlotLane, prep_bytes + (pair ? 20.0 * P : 0), P);
+      LaunchPrepCount(pc, st_);
+    } else {
+      Timed t(ctx_, st_, kSlotPrep, prep_bytes, P);
+      LaunchPrepCompute(pa, st_);
+    }
+    Scan(size.get(), P, offs.get());
+    const uint64_t W = ReadU64(offs.get() + P);
+    if (lane_count && W == 0) return;
 
-    if (nlists == 1 && nne == 0 && !pair) {
+    if (zero_copy) {
       // the candidate set is a window of one adjacency list: no copy
       if (op.kind == kComputeCount) {
         if (!bk_.bounds) {
@@ -724,7 +768,7 @@ class Runner {
         it.acc_level = acc_ + 1 + L;
         it.bk = bk_;
         Timed t(ctx_, st_, kSlotIntersect, 4.0 * nlists * items + seg_bytes * nseg + (pair ? 20.0 * nseg : 0), items);
-        LaunchVerify(it, pair ? kModeSumPair : kModeSum, st_);
+        LaunchVerify(it, pair ? kModeSumPair : kModeSum, ctx_->tuning.item_grid, st_);
         l = r;
         il = ir;
         continue;
@@ -735,7 +779,7 @@ class Runner {
       it.ccount = ccount.get();
       {
         Timed t(ctx_, st_, kSlotIntersect, 4.0 * nlists * items + seg_bytes * nseg + 12.0 * nchunks, items);
-        LaunchVerify(it, kModeMask, st_);
+        LaunchVerify(it, kModeMask, ctx_->tuning.item_grid, st_);
       }
       Buf<uint64_t> cbase(pool_, nchunks + 1);
       Scan32(ccount.get(), nchunks, cbase.get());
@@ -834,7 +878,7 @@ class Runner {
         it.ccount = ccount.get();
         {
           Timed t(ctx_, st_, kSlotFilter, 4.0 * items + (16.0 + 4.0 * nne) * nseg + 12.0 * nchunks, items);
-          LaunchVerify(it, kModeMask, st_);
+          LaunchVerify(it, kModeMask, ctx_->tuning.item_grid, st_);
         }
         Scan32(ccount.get(), nchunks, cbase.get());
         C = ReadU64(cbase.get() + nchunks);
@@ -961,7 +1005,7 @@ class Runner {
       it.ccount = ccount.get();
       {
         Timed t(ctx_, st_, kSlotFilter, 4.0 * items + (16.0 + 4.0 * nne) * nseg + 12.0 * nchunks, items);
-        LaunchVerify(it, kModeMask, st_);
+        LaunchVerify(it, kModeMask, ctx_->tuning.item_grid, st_);
       }
       Buf<uint64_t> cbase(pool_, nchunks + 1);
       Scan32(ccount.get(), nchunks, cbase.get());
@@ -1041,7 +1085,7 @@ class Runner {
       it.acc_level = acc_ + 1 + L;
       it.bk = bk_;
       Timed t(ctx_, st_, kSlotCount, 8.0 * items + 40.0 * nseg, items);
-      LaunchVerify(it, kModeSumTwo, st_);
+      LaunchVerify(it, kModeSumTwo, ctx_->tuning.item_grid, st_);
       l = r;
       il = ir;
     }
@@ -1053,6 +1097,7 @@ class Runner {
   hipStream_t st_;
   DevicePool* pool_;
   unsigned long long* acc_ = nullptr;
+  int lane_lists_[kMaxOps];  // lists per item of lane-counted levels
   uint64_t host_level_[kMaxOps];
   uint64_t levels_[kMaxOps];
   uint64_t host_count_ = 0;

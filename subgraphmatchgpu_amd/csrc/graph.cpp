@@ -4,10 +4,11 @@
 #include <omp.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <fstream>
+#include <iterator>
 #include <sstream>
-#include <unordered_set>
 
 namespace sme {
 
@@ -245,22 +246,105 @@ void GenerateRmat(int scale, int edge_factor, double a, double b, double c, uint
   BuildSimpleUndirected(n, packed, g);
 }
 
+namespace {
+
+// First m distinct unordered pairs of the stream (x, y) = (Next() % n,
+// Next() % n), x != y, as sorted keys (min << 32) | max.  Drawing exactly the
+// current deficit per round and de-duplicating keeps the set identical to a
+// one-at-a-time rejection loop over the same stream.
+void DrawDistinctPairs(uint64_t n, uint64_t m, Xoshiro256ss* rng, std::vector<uint64_t>* keys) {
+  keys->clear();
+  std::vector<uint64_t> fresh, merged;
+  while (keys->size() < m) {
+    const uint64_t want = m - keys->size();
+    fresh.clear();
+    for (uint64_t i = 0; i < want; ++i) {
+      uint64_t x = rng->Next() % n;
+      uint64_t y = rng->Next() % n;
+      if (x == y) continue;
+      if (x > y) std::swap(x, y);
+      fresh.push_back((x << 32) | y);
+    }
+    std::sort(fresh.begin(), fresh.end());
+    fresh.erase(std::unique(fresh.begin(), fresh.end()), fresh.end());
+    merged.clear();
+    std::set_union(keys->begin(), keys->end(), fresh.begin(), fresh.end(),
+                   std::back_inserter(merged));
+    keys->swap(merged);
+  }
+}
+
+// CSR of the simple graph given by sorted distinct keys (x << 32) | y, x < y,
+// written at vertex offset `vbase` into rp[0..n) (entry offsets starting at
+// `ebase`; the caller sets the end offset) and cols.  Filling rows in key order leaves every list sorted:
+// neighbours below u are placed before those above it, each in order.
+void FillSortedPairs(uint64_t n, const std::vector<uint64_t>& keys, uint64_t vbase, uint64_t ebase,
+                     uintE* rp, uintV* cols) {
+  std::vector<uint64_t> cur(n + 1, 0);
+  for (uint64_t k : keys) {
+    cur[(k >> 32) + 1]++;
+    cur[(k & 0xffffffffull) + 1]++;
+  }
+  for (uint64_t u = 0; u < n; ++u) cur[u + 1] += cur[u];
+  for (uint64_t u = 0; u < n; ++u) rp[u] = ebase + cur[u];
+  for (uint64_t k : keys) {
+    const uint64_t x = k >> 32, y = k & 0xffffffffull;
+    cols[ebase + cur[x]++] = static_cast<uintV>(vbase + y);
+    cols[ebase + cur[y]++] = static_cast<uintV>(vbase + x);
+  }
+}
+
+}  // namespace
+
 void GenerateErdosRenyi(uint64_t n, uint64_t m, uint64_t seed, HostGraph* g) {
   const uint64_t max_edges = n * (n - 1) / 2;
   if (m > max_edges) m = max_edges;
   Xoshiro256ss rng(seed);
-  std::unordered_set<uint64_t> seen;
-  seen.reserve(m * 2);
-  std::vector<uint64_t> packed;
-  packed.reserve(m);
-  while (packed.size() < m) {
-    uint64_t x = rng.Next() % n, y = rng.Next() % n;
-    if (x == y) continue;
-    if (x > y) std::swap(x, y);
-    const uint64_t key = (x << 32) | y;
-    if (seen.insert(key).second) packed.push_back(key);
+  std::vector<uint64_t> keys;
+  DrawDistinctPairs(n, m, &rng, &keys);
+  g->n = n;
+  g->nnz = 2 * keys.size();
+  g->row_ptrs.assign(n + 1, 0);
+  g->cols.resize(g->nnz);
+  FillSortedPairs(n, keys, 0, 0, g->row_ptrs.data(), g->cols.data());
+  g->row_ptrs[n] = g->nnz;
+}
+
+void GenerateErdosRenyiBatch(uint64_t ngraphs, uint64_t n_min, uint64_t n_max, double deg_min,
+                             double deg_max, uint64_t seed, HostGraph* g,
+                             std::vector<uint64_t>* graph_offs) {
+  // parameters of every member from one master stream, in member order
+  std::vector<uint64_t> nk(ngraphs), mk(ngraphs), sk(ngraphs);
+  Xoshiro256ss master(seed);
+  for (uint64_t k = 0; k < ngraphs; ++k) {
+    nk[k] = n_min + master.Next() % (n_max - n_min + 1);
+    const double deg = deg_min + (deg_max - deg_min) * master.Uniform();
+    const uint64_t max_edges = nk[k] * (nk[k] - 1) / 2;
+    mk[k] = std::min<uint64_t>(max_edges, static_cast<uint64_t>(std::llround(nk[k] * deg / 2.0)));
+    sk[k] = master.Next();
   }
-  BuildSimpleUndirected(n, packed, g);
+  graph_offs->assign(ngraphs + 1, 0);
+  std::vector<uint64_t> eoffs(ngraphs + 1, 0);
+  for (uint64_t k = 0; k < ngraphs; ++k) {
+    (*graph_offs)[k + 1] = (*graph_offs)[k] + nk[k];
+    eoffs[k + 1] = eoffs[k] + 2 * mk[k];
+  }
+  g->n = (*graph_offs)[ngraphs];
+  g->nnz = eoffs[ngraphs];
+  g->row_ptrs.assign(g->n + 1, 0);
+  g->cols.resize(g->nnz);
+#pragma omp parallel
+  {
+    std::vector<uint64_t> keys;
+#pragma omp for schedule(dynamic, 4)
+    for (int64_t k = 0; k < static_cast<int64_t>(ngraphs); ++k) {
+      Xoshiro256ss rng(sk[k]);
+      DrawDistinctPairs(nk[k], mk[k], &rng, &keys);
+      FillSortedPairs(nk[k], keys, (*graph_offs)[k], eoffs[k], g->row_ptrs.data() + (*graph_offs)[k],
+                      g->cols.data());
+    }
+  }
+  g->row_ptrs[g->n] = g->nnz;
 }
 
 }  // namespace sme

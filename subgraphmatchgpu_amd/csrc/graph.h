@@ -49,6 +49,17 @@ void GenerateRmat(int scale, int edge_factor, double a, double b, double c, uint
 // Seeded Erdos-Renyi G(n, m): m distinct undirected edges drawn by rejection.
 void GenerateErdosRenyi(uint64_t n, uint64_t m, uint64_t seed, HostGraph* g);
 
+// `ngraphs` independent G(n, m) graphs packed back to back into one CSR (a
+// disjoint union).  Member k has n_k uniform in [n_min, n_max] vertices and
+// m_k = round(n_k * d_k / 2) edges for an average degree d_k uniform in
+// [deg_min, deg_max); (n_k, d_k, seed_k) come from one master xoshiro256**
+// stream in member order, and member k is GenerateErdosRenyi(n_k, m_k,
+// seed_k) shifted by graph_offs[k].  graph_offs gets ngraphs + 1 vertex
+// offsets.
+void GenerateErdosRenyiBatch(uint64_t ngraphs, uint64_t n_min, uint64_t n_max, double deg_min,
+                             double deg_max, uint64_t seed, HostGraph* g,
+                             std::vector<uint64_t>* graph_offs);
+
 // Deterministic 64-bit PRNG pair used by the generators.
 struct SplitMix64 {
   uint64_t s;

@@ -129,6 +129,18 @@ struct ItemArgs {
   BucketRef bk;                   // counting modes only
 };
 
+// COMPUTE_COUNT set-up with lane-per-path counting of short paths: paths
+// whose pivot window holds at most lane_max items are verified in the set-up
+// kernel by one lane each (their size becomes 0); `it` carries the membership
+// data indexed by path (cs / reuse / pair of the whole frame).
+struct PrepCountArgs {
+  PrepComputeArgs pa;
+  ItemArgs it;
+  uint64_t lane_max;
+  uint8_t pair;                  // COMPUTE_COUNT with one other unmaterialized vertex
+  unsigned long long* lane_items;  // out (optional): items verified by lanes
+};
+
 enum VerifyMode : int { kModeMask = 0, kModeSum = 1, kModeSumPair = 2, kModeSumTwo = 3 };
 
 // Rank structures built from a verify pass.
@@ -187,12 +199,15 @@ struct PrepTwoArgs {
 // ---------------------------------------------------------------------------
 // Launchers (all on `stream`).
 void LaunchPrepCompute(const PrepComputeArgs& a, hipStream_t stream);
+void LaunchPrepCount(const PrepCountArgs& a, hipStream_t stream);
 void LaunchPrepList(const PrepListArgs& a, hipStream_t stream);
 void LaunchPrepTwo(const PrepTwoArgs& a, hipStream_t stream);
 void LaunchCountPaths(const CountPathArgs& a, hipStream_t stream);
 void LaunchChunkPartition(const uint64_t* offs, uint64_t nseg, uint64_t nchunks,
                           uint32_t* chunk_seg, hipStream_t stream);
-void LaunchVerify(const ItemArgs& a, int mode, hipStream_t stream);
+// max_grid caps the workgroups of the launch (0: default cap); every wave
+// loops over chunks, so any cap >= 1 covers all items.
+void LaunchVerify(const ItemArgs& a, int mode, uint32_t max_grid, hipStream_t stream);
 void LaunchCompact(const CompactArgs& a, hipStream_t stream);
 void LaunchExpand(const ExpandArgs& a, hipStream_t stream);
 constexpr uint32_t kNoHub = 0xffffffffu;

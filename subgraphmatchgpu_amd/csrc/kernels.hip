@@ -111,69 +111,81 @@ __device__ __forceinline__ uint64_t GridStride() {
   return static_cast<uint64_t>(gridDim.x) * blockDim.x;
 }
 
-uint64_t GridFor(uint64_t n, uint64_t per_block) {
+uint64_t GridFor(uint64_t n, uint64_t per_block, uint64_t cap = kMaxGrid) {
   uint64_t g = (n + per_block - 1) / per_block;
   if (g < 1) g = 1;
-  return g > kMaxGrid ? kMaxGrid : g;
+  if (cap == 0 || cap > kMaxGrid) cap = kMaxGrid;
+  return g > cap ? cap : g;
 }
 
 // --------------------------------------------------------------------------
+// Set-up of one path of an intersection: the admissible window [lo, hi)
+// from the < / > conditions, the pivot list and the pivot's window [*b, *e).
+__device__ __forceinline__ void PrepPath(const PrepComputeArgs& a, uint64_t p, uint64_t* pb,
+                                         uint64_t* pe, uint8_t* ppiv) {
+  uint64_t lo = 0, hi = 0x100000000ull;
+  for (int i = 0; i < a.ngt; ++i) {
+    const uint64_t v = Gather(a.cs, a.gt_v[i], p) + 1ull;
+    lo = v > lo ? v : lo;
+  }
+  for (int i = 0; i < a.nlt; ++i) {
+    const uint64_t v = Gather(a.cs, a.lt_v[i], p);
+    hi = v < hi ? v : hi;
+  }
+  // pivot: list of minimum length (ThreadChoosePivotLevel,
+  // CheckConstraintsCommon.cuh:10-29; first minimum wins).  A reused
+  // parent segment is short, so it is cut to the window before comparing.
+  uint64_t best_b = 0, best_e = 0;
+  uint8_t best = 0;
+  bool have = false, windowed = false;
+  if (a.has_reuse) {
+    const uint64_t seg = SegIndex(a.reuse, p);
+    uint64_t b = a.reuse.beg[seg], e = b + a.reuse.len[seg];
+    if (lo >= hi) {
+      e = b;
+    } else {
+      if (lo > 0) b = LowerBound(a.reuse.base, b, e, static_cast<uint32_t>(lo));
+      if (hi <= 0xffffffffull) e = LowerBound(a.reuse.base, b, e, static_cast<uint32_t>(hi));
+    }
+    best = a.nb;
+    best_b = b;
+    best_e = e;
+    have = windowed = true;
+  }
+  for (int j = 0; j < a.nb; ++j) {
+    const uint32_t v = Gather(a.cs, a.bconn[j], p);
+    const uint64_t b = a.rp[v], e = a.rp[v + 1];
+    if (!have || e - b < best_e - best_b) {
+      best = static_cast<uint8_t>(j);
+      best_b = b;
+      best_e = e;
+      have = true;
+      windowed = false;
+    }
+  }
+  uint64_t b = best_b, e = best_e;
+  if (!windowed) {
+    if (lo >= hi) {
+      e = b;
+    } else {
+      if (lo > 0) b = LowerBound(a.cols, b, e, static_cast<uint32_t>(lo));
+      if (hi <= 0xffffffffull) e = LowerBound(a.cols, b, e, static_cast<uint32_t>(hi));
+    }
+  }
+  *pb = b;
+  *pe = e;
+  *ppiv = best;
+}
+
 __global__ __launch_bounds__(kThreads) void PrepComputeKernel(PrepComputeArgs a) {
   for (uint64_t p = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; p < a.npaths;
        p += GridStride()) {
-    // window [lo, hi) of admissible candidate values
-    uint64_t lo = 0, hi = 0x100000000ull;
-    for (int i = 0; i < a.ngt; ++i) {
-      const uint64_t v = Gather(a.cs, a.gt_v[i], p) + 1ull;
-      lo = v > lo ? v : lo;
-    }
-    for (int i = 0; i < a.nlt; ++i) {
-      const uint64_t v = Gather(a.cs, a.lt_v[i], p);
-      hi = v < hi ? v : hi;
-    }
-    // pivot: list of minimum length (ThreadChoosePivotLevel,
-    // CheckConstraintsCommon.cuh:10-29; first minimum wins).  A reused
-    // parent segment is short, so it is cut to the window before comparing.
-    uint64_t best_b = 0, best_e = 0;
-    uint8_t best = 0;
-    bool have = false, windowed = false;
-    if (a.has_reuse) {
-      const uint64_t seg = SegIndex(a.reuse, p);
-      uint64_t b = a.reuse.beg[seg], e = b + a.reuse.len[seg];
-      if (lo >= hi) {
-        e = b;
-      } else {
-        if (lo > 0) b = LowerBound(a.reuse.base, b, e, static_cast<uint32_t>(lo));
-        if (hi <= 0xffffffffull) e = LowerBound(a.reuse.base, b, e, static_cast<uint32_t>(hi));
-      }
-      best = a.nb;
-      best_b = b;
-      best_e = e;
-      have = windowed = true;
-    }
-    for (int j = 0; j < a.nb; ++j) {
-      const uint32_t v = Gather(a.cs, a.bconn[j], p);
-      const uint64_t b = a.rp[v], e = a.rp[v + 1];
-      if (!have || e - b < best_e - best_b) {
-        best = static_cast<uint8_t>(j);
-        best_b = b;
-        best_e = e;
-        have = true;
-        windowed = false;
-      }
-    }
-    uint64_t b = best_b, e = best_e;
-    if (!windowed) {
-      if (lo >= hi) {
-        e = b;
-      } else {
-        if (lo > 0) b = LowerBound(a.cols, b, e, static_cast<uint32_t>(lo));
-        if (hi <= 0xffffffffull) e = LowerBound(a.cols, b, e, static_cast<uint32_t>(hi));
-      }
-    }
+    uint64_t b, e;
+    uint8_t pv;
+    PrepPath(a, p, &b, &e, &pv);
     a.size[p] = e - b;
     a.pstart[p] = b;
-    if (a.piv) a.piv[p] = best;
+    if (a.piv) a.piv[p] = pv;
   }
 }
 
@@ -286,9 +298,8 @@ __global__ __launch_bounds__(kThreads) void ChunkPartitionKernel(const uint64_t*
 // Membership + non-equality test of one item (the verify functor of
 // GpsmBinSearch, GPSMBinSearch.cuh:146-170, with the < / > part already
 // applied through the window).
-__device__ __forceinline__ bool ItemValid(const ItemArgs& a, uint64_t s, uint32_t x) {
+__device__ __forceinline__ bool ItemValidPv(const ItemArgs& a, uint64_t s, uint8_t pv, uint32_t x) {
   if (a.nb + a.has_reuse > 1) {
-    const uint8_t pv = a.piv[s];
     for (int j = 0; j < a.nb; ++j) {
       if (j == pv) continue;
       const uint32_t v = Gather(a.cs, a.bconn[j], s);
@@ -308,6 +319,10 @@ __device__ __forceinline__ bool ItemValid(const ItemArgs& a, uint64_t s, uint32_
   for (int i = 0; i < a.nne; ++i)
     if (x == Gather(a.cs, a.ne_v[i], s)) return false;
   return true;
+}
+
+__device__ __forceinline__ bool ItemValid(const ItemArgs& a, uint64_t s, uint32_t x) {
+  return ItemValidPv(a, s, a.nb + a.has_reuse > 1 ? a.piv[s] : 0, x);
 }
 
 // Values of segment s come from the graph's lists or, when the pivot is the
@@ -379,6 +394,61 @@ __global__ __launch_bounds__(kThreads) void VerifyKernel(ItemArgs a) {
       atomicAdd(a.acc, local);
       atomicAdd(a.acc_level, local);
     }
+  }
+}
+
+// COMPUTE_COUNT set-up that also counts short paths: a path whose pivot
+// window holds at most lane_max items is verified by its own lane here and
+// leaves size 0, so only longer paths go through the item schedule.  The
+// loop is wave-uniform so per-bucket sums can use the whole wave.
+__global__ __launch_bounds__(kThreads) void PrepCountKernel(PrepCountArgs a) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t np = a.pa.npaths;
+  u64 local = 0, nlane = 0;
+  for (uint64_t base = blockIdx.x * static_cast<uint64_t>(blockDim.x) + (threadIdx.x - lane);
+       base < np; base += GridStride()) {
+    const uint64_t p = base + lane;
+    u64 val = 0;
+    if (p < np) {
+      uint64_t b, e;
+      uint8_t pv;
+      PrepPath(a.pa, p, &b, &e, &pv);
+      uint64_t n = e - b;
+      if (n <= a.lane_max) {
+        const uint32_t* src = (a.it.has_reuse && pv == a.it.nb) ? a.it.reuse.base : a.it.cols;
+        uint64_t ps = 0, plen = 0;
+        const uint32_t* parr = nullptr;
+        if (a.pair) {
+          ps = SegIndex(a.it.pair, p);
+          parr = a.it.pair.base + a.it.pair.beg[ps];
+          plen = a.it.pair.len[ps];
+        }
+        for (uint64_t i = b; i < e; ++i) {
+          const uint32_t x = src[i];
+          if (!ItemValidPv(a.it, p, pv, x)) continue;
+          val += a.pair ? CountWithComputed(parr, plen, x, a.it.pair_op) : 1ull;
+        }
+        nlane += n;
+        n = 0;
+      }
+      a.pa.size[p] = n;
+      a.pa.pstart[p] = b;
+      if (a.pa.piv) a.pa.piv[p] = pv;
+    }
+    local += val;
+    if (a.it.bk.bounds) {
+      const u64 key = p < np ? BucketOf(a.it.bk, Gather(a.it.cs, a.it.bk.start_v, p)) : kNoBucket;
+      BucketAdd(a.it.bk, key, val);
+    }
+  }
+  local = WaveSum(local);
+  nlane = WaveSum(nlane);
+  if (lane == 0) {
+    if (local) {
+      atomicAdd(a.it.acc, local);
+      atomicAdd(a.it.acc_level, local);
+    }
+    if (nlane && a.lane_items) atomicAdd(a.lane_items, nlane);
   }
 }
 
@@ -607,6 +677,11 @@ void LaunchPrepCompute(const PrepComputeArgs& a, hipStream_t stream) {
   hipLaunchKernelGGL(PrepComputeKernel, dim3(GridFor(a.npaths, kThreads)), dim3(kThreads), 0, stream, a);
 }
 
+void LaunchPrepCount(const PrepCountArgs& a, hipStream_t stream) {
+  if (a.pa.npaths == 0) return;
+  hipLaunchKernelGGL(PrepCountKernel, dim3(GridFor(a.pa.npaths, kThreads)), dim3(kThreads), 0, stream, a);
+}
+
 void LaunchPrepList(const PrepListArgs& a, hipStream_t stream) {
   if (a.npaths == 0) return;
   hipLaunchKernelGGL(PrepListKernel, dim3(GridFor(a.npaths, kThreads)), dim3(kThreads), 0, stream, a);
@@ -628,9 +703,9 @@ void LaunchChunkPartition(const uint64_t* offs, uint64_t nseg, uint64_t nchunks,
   hipLaunchKernelGGL(ChunkPartitionKernel, dim3(GridFor(nseg, kThreads)), dim3(kThreads), 0, stream, offs, nseg, nchunks, chunk_seg);
 }
 
-void LaunchVerify(const ItemArgs& a, int mode, hipStream_t stream) {
+void LaunchVerify(const ItemArgs& a, int mode, uint32_t max_grid, hipStream_t stream) {
   if (a.nchunks == 0) return;
-  const uint64_t grid = GridFor(a.nchunks, kWavesPerBlock);
+  const uint64_t grid = GridFor(a.nchunks, kWavesPerBlock, max_grid);
   switch (mode) {
     case kModeMask: hipLaunchKernelGGL(VerifyKernel<kModeMask>, dim3(grid), dim3(kThreads), 0, stream, a); break;
     case kModeSum: hipLaunchKernelGGL(VerifyKernel<kModeSum>, dim3(grid), dim3(kThreads), 0, stream, a); break;

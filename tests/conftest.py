@@ -41,3 +41,14 @@ def clique(k):
     import subgraphmatchgpu_amd as sme
 
     return sme.from_adjacency([[j for j in range(k) if j != i] for i in range(k)])
+
+
+def member(g, offs, k):
+    """Member k of a packed batch as a standalone graph (ids from 0)."""
+    import numpy as np
+    import subgraphmatchgpu_amd as sme
+
+    a, b = int(offs[k]), int(offs[k + 1])
+    rp = g.row_ptrs[a:b + 1].astype(np.uint64)
+    cols = g.cols[int(rp[0]):int(rp[-1])].astype(np.int64) - a
+    return sme.Graph(rp - rp[0], cols.astype(np.uint32))

@@ -119,3 +119,47 @@ def test_emu_clique_without_ordering(emu, graphs):
     for q in (0, 3, 7):
         want = oracle.light_count(g.row_ptrs, g.cols, q, 3, enable_ordering=False)
         assert run_emu(emu, p, 6, q, 3, 0, g.n, 1 << 30, ordering=0)[:2] == (want[0], want[1])
+
+
+def test_emu_er_batch_members(emu, tmp_path):
+    """Per-member counts of a packed batch of ER graphs in one pass equal the
+    WCOJ oracle on each member alone."""
+    from conftest import member
+
+    G, offs = sme.gen_er_batch(12, (8, 40), (3.0, 10.0), seed=3)
+    p = str(tmp_path / "batch.bin")
+    sme.write_bin(p, G)
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0", BUCKETS=",".join(map(str, offs)))
+    for q in (0, 1, 2, 3):
+        want = [oracle.wcoj_count(m.row_ptrs, m.cols, q)
+                for m in (member(G, offs, k) for k in range(12))]
+        r = subprocess.run([emu, p, "6", str(q), "3", "1", "0", str(G.n), str(1 << 30)],
+                           capture_output=True, text=True, env=env, timeout=600)
+        assert r.returncode == 0, r.stderr[-2000:]
+        got = [int(x) for x in r.stdout.split("buckets=")[1].split(",")]
+        assert got == want, q
+        assert any(want)
+
+
+@pytest.mark.parametrize("tune", ["4,0,0", "64,3,0", "1000000,1,20000"])
+def test_emu_tunings_do_not_change_results(emu, graphs, tune):
+    """Lane-per-path counting, tiny grids and tiny batch caps: same counts,
+    levels and per-bucket counts."""
+    g, p = graphs["rmat7"]
+    bounds = [0, 3, 17, 40, 90, g.n]
+    for q in (0, 1, 2, 3, 4, 7, 13):
+        want, wl = oracle.light_count(g.row_ptrs, g.cols, q, 3)
+        env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0", TUNE=tune)
+        r = subprocess.run([emu, p, "6", str(q), "3", "1", "0", str(g.n), str(1 << 30)],
+                           capture_output=True, text=True, env=env, timeout=600)
+        assert r.returncode == 0, r.stderr[-2000:]
+        parts = r.stdout.split()
+        assert int(parts[0].split("=")[1]) == want
+        assert [int(x) for x in parts[1].split("=")[1].split(",")] == wl
+        env["BUCKETS"] = ",".join(map(str, bounds))
+        r = subprocess.run([emu, p, "6", str(q), "3", "1", "0", str(g.n), str(1 << 30)],
+                           capture_output=True, text=True, env=env, timeout=600)
+        assert r.returncode == 0, r.stderr[-2000:]
+        got = [int(x) for x in r.stdout.split("buckets=")[1].split(",")]
+        assert got == [oracle.light_count(g.row_ptrs, g.cols, q, 3, v_begin=a, v_end=b)[0]
+                       for a, b in zip(bounds[:-1], bounds[1:])], q

@@ -197,3 +197,62 @@ def test_self_loops_handled(dev):
         got = dev.count(dg, sme.Plan(q))
         assert (got.count, got.levels) == lw
         assert got.count == want
+
+
+@pytest.mark.parametrize("tune", [dict(lane_path_max=8), dict(lane_path_max=1 << 20, item_grid=64),
+                                  dict(item_grid=1, batch_bytes=1 << 16)])
+def test_tunings_do_not_change_results(fixture_graphs, tune):
+    g = fixture_graphs["rmat10"]
+    d = sme.Device(0, budget_bytes=2 << 30)
+    d.set_tuning(**tune)
+    assert {k: d.get_tuning()[k] for k in tune} == tune
+    dg = d.upload(g)
+    bounds = [0, 5, 100, 400, g.n]
+    for q in range(14):
+        say(f"case tuning {tune} q={q}")
+        want = oracle.light_count(g.row_ptrs, g.cols, q, 3)
+        r = d.count(dg, sme.Plan(q))
+        assert (r.count, r.levels) == want
+        if q in (0, 1, 3, 4, 7):
+            got, _ = d.count_buckets(dg, sme.Plan(q), bounds)
+            assert got.tolist() == [oracle.light_count(g.row_ptrs, g.cols, q, 3, v_begin=a, v_end=b)[0]
+                                    for a, b in zip(bounds[:-1], bounds[1:])]
+    d.close()
+
+
+def test_er_batch_members_one_pass(dev):
+    """Packed ER graphs: one bucketed pass gives each member's triangle and
+    4-cycle count, equal to the WCOJ oracle on the member alone and to one
+    pass per member."""
+    from conftest import member
+
+    G, offs = sme.gen_er_batch(300, (64, 512), (4.0, 48.0), seed=11)
+    dg = dev.upload(G)
+    for q in (sme.Q0, sme.Q1):
+        say(f"case er-batch q={q}")
+        got, whole = dev.count_buckets(dg, sme.Plan(q), offs)
+        want = [oracle.wcoj_count(m.row_ptrs, m.cols, q) for m in (member(G, offs, k) for k in range(300))]
+        assert got.tolist() == want
+        assert whole.count == sum(want)
+        for k in range(0, 300, 37):
+            assert dev.count(dg, sme.Plan(q), int(offs[k]), int(offs[k + 1])).count == want[k]
+    dg.release()
+
+
+def test_autotune_installs_a_measured_setting(fixture_graphs):
+    g = fixture_graphs["rmat12"]
+    d = sme.Device(0, budget_bytes=2 << 30)
+    dg = d.upload(g)
+    plans = [sme.Plan(sme.Q3), sme.Plan(sme.Q2)]
+    want = [d.count(dg, p).count for p in plans]
+    say("case autotune rmat12")
+    best, ms, trials = d.autotune(dg, plans, sample_fraction=0.05)
+    assert d.get_tuning() == best
+    assert len(trials) >= 8 and ms == min(t["ms"] for t in trials)
+    assert [d.count(dg, p).count for p in plans] == want
+    G, offs = sme.gen_er_batch(64, (64, 256), (4.0, 16.0), seed=2)
+    dgb = d.upload(G)
+    say("case autotune er-batch")
+    best, ms, trials = d.autotune(dgb, [sme.Plan(sme.Q0), sme.Plan(sme.Q1)], bounds=offs, sample_fraction=0.25)
+    assert d.get_tuning() == best
+    d.close()

@@ -117,3 +117,42 @@ def test_cli_usage_runs():
 
     r = subprocess.run([sme.CLI_PATH], capture_output=True, text=True)
     assert r.returncode == 1 and "patternmatchgpu" in r.stdout
+
+
+def test_er_batch_generator_layout():
+    from conftest import member
+
+    G, offs = sme.gen_er_batch(64, (16, 200), (4.0, 20.0), seed=7)
+    assert offs[0] == 0 and offs[-1] == G.n and len(offs) == 65
+    sizes = np.diff(offs.astype(np.int64))
+    assert sizes.min() >= 16 and sizes.max() <= 200
+    assert len(set(sizes.tolist())) > 10
+    rp, cols = G.row_ptrs, G.cols
+    for k in range(64):
+        a, b = int(offs[k]), int(offs[k + 1])
+        seg = cols[int(rp[a]):int(rp[b])]
+        assert seg.size == 0 or (seg.min() >= a and seg.max() < b)  # disjoint union
+        m = member(G, offs, k)
+        deg = 2 * (m.nnz // 2) / m.n
+        assert 4.0 - 1.0 / m.n <= deg <= 20.0 + 1.0 / m.n
+        src = np.repeat(np.arange(m.n), np.diff(m.row_ptrs.astype(np.int64)))
+        edges = set(zip(src.tolist(), m.cols.tolist()))
+        assert all((y, x) in edges for x, y in edges) and all(x != y for x, y in edges)
+        for u in range(m.n):
+            assert np.all(np.diff(m.cols[m.row_ptrs[u]:m.row_ptrs[u + 1]].astype(np.int64)) > 0)
+    # seeded and independent of the thread count
+    G2, offs2 = sme.gen_er_batch(64, (16, 200), (4.0, 20.0), seed=7)
+    assert G2.checksum() == G.checksum() and np.array_equal(offs2, offs)
+    import subprocess
+    import sys
+
+    code = ("import subgraphmatchgpu_amd as s; g, o = s.gen_er_batch(64, (16, 200), (4.0, 20.0), seed=7);"
+            "print(g.checksum())")
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, check=True,
+                         env=dict(os.environ, OMP_NUM_THREADS="1"), cwd=ROOT).stdout.strip()
+    assert out == G.checksum()
+    assert sme.gen_er_batch(64, (16, 200), (4.0, 20.0), seed=8)[0].checksum() != G.checksum()
+    with pytest.raises(sme.SmeError):
+        sme.gen_er_batch(0)
+    with pytest.raises(sme.SmeError):
+        sme.gen_er_batch(4, (100, 50))
