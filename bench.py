@@ -89,7 +89,7 @@ def main():
                     help="members timed with one pass each (er-batch)")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tune", default="",
-                    help="lane_path_max,item_grid,batch_bytes schedule setting (rmat20)")
+                    help="lane_path_max,item_grid,batch_bytes,super_chunks schedule setting (rmat20)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU; --gpus must be 1"}))
@@ -116,8 +116,8 @@ def main():
     log(f"upload: {1e3 * (time.perf_counter() - t0):.1f} ms, {dg.hubs} hub bitmaps")
     plans = [sme.Plan(q, sme.GPU_LIGHT, sme.CL_SPECIAL) for q in QUERIES]
     if args.tune:
-        lane, grid, batch = (int(x) for x in args.tune.split(","))
-        dev.set_tuning(lane_path_max=lane, item_grid=grid, batch_bytes=batch)
+        f = [int(x) for x in args.tune.split(",")] + [0, 0, 0, 0]
+        dev.set_tuning(lane_path_max=f[0], item_grid=f[1], batch_bytes=f[2], super_chunks=f[3])
     log(f"schedule: {dev.get_tuning()}")
 
     def step():

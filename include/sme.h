@@ -148,6 +148,10 @@ int sme_count_buckets(sme_ctx* ctx, const sme_graph* g, const sme_plan* p, const
  *   item_grid      workgroup cap of the candidate kernels (0: 16384); every
  *                  wavefront loops over 64-candidate chunks, so a cap near
  *                  the resident capacity makes the launch persistent.
+ *   super_chunks   64-candidate chunks a wavefront takes at once; when they
+ *                  all belong to one path, that path's lists are loaded once
+ *                  and the window of another sorted list that the chunks'
+ *                  values span is staged in LDS (0 or 1: chunk by chunk).
  *   batch_bytes    cap on the device bytes one batch of a level may use
  *                  (0: the level's share of the memory budget).
  * Settings apply to every later call on the context; results never depend
@@ -155,7 +159,7 @@ int sme_count_buckets(sme_ctx* ctx, const sme_graph* g, const sme_plan* p, const
 typedef struct {
   uint64_t lane_path_max;
   uint32_t item_grid;
-  uint32_t reserved;
+  uint32_t super_chunks;
   uint64_t batch_bytes;
 } sme_tuning;
 int sme_set_tuning(sme_ctx* ctx, const sme_tuning* t);
@@ -167,8 +171,8 @@ typedef struct {
 } sme_tune_trial;
 
 /* Autotuner: times the plans on a sample of the start vertices for each
- * candidate setting (coordinate descent over lane_path_max, item_grid and
- * batch_bytes, starting from the context's current setting), installs the
+ * candidate setting (coordinate descent over lane_path_max, super_chunks,
+ * item_grid and batch_bytes, starting from the context's current setting), installs the
  * fastest on the context and returns it.  With bounds (nbuckets >= 1) the
  * sample is 8 evenly spaced blocks of consecutive buckets covering about
  * sample_fraction of the buckets, run through sme_count_buckets; without

@@ -54,11 +54,11 @@ class _Stats(ctypes.Structure):
 
 class _Tuning(ctypes.Structure):
     _fields_ = [("lane_path_max", ctypes.c_uint64), ("item_grid", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32), ("batch_bytes", ctypes.c_uint64)]
+                ("super_chunks", ctypes.c_uint32), ("batch_bytes", ctypes.c_uint64)]
 
     def as_dict(self) -> dict:
         return {"lane_path_max": int(self.lane_path_max), "item_grid": int(self.item_grid),
-                "batch_bytes": int(self.batch_bytes)}
+                "super_chunks": int(self.super_chunks), "batch_bytes": int(self.batch_bytes)}
 
 
 class _Trial(ctypes.Structure):
@@ -345,13 +345,16 @@ class Device:
         uploaded afterwards (0 disables)."""
         _check(library().sme_set_hub_bitmaps(self._h, min_degree, max_bytes), self._h)
 
-    def set_tuning(self, lane_path_max: int = 0, item_grid: int = 0, batch_bytes: int = 0):
+    def set_tuning(self, lane_path_max: int = 0, item_grid: int = 0, batch_bytes: int = 0,
+                   super_chunks: int = 0):
         """Schedule tunables for later calls (results never depend on them):
         lane_path_max -- counting-level paths with at most this many pivot
         candidates are verified by one lane each; item_grid -- workgroup cap
         of the candidate kernels (0: default); batch_bytes -- cap on one
-        batch's device bytes (0: the level's share of the budget)."""
-        t = _Tuning(lane_path_max, item_grid, 0, batch_bytes)
+        batch's device bytes (0: the level's share of the budget);
+        super_chunks -- 64-candidate chunks per wavefront step (runs inside
+        one path load the path once and stage a list window in LDS)."""
+        t = _Tuning(lane_path_max, item_grid, super_chunks, batch_bytes)
         _check(library().sme_set_tuning(self._h, ctypes.byref(t)), self._h)
 
     def get_tuning(self) -> dict:

@@ -206,6 +206,7 @@ int sme_set_tuning(sme_ctx* ctx, const sme_tuning* t) {
   if (!ctx || !t) return Report(ctx, SME_ERR_INVALID, "null argument");
   ctx->impl.tuning.lane_path_max = t->lane_path_max;
   ctx->impl.tuning.item_grid = t->item_grid;
+  ctx->impl.tuning.super_chunks = t->super_chunks;
   ctx->impl.tuning.batch_bytes = t->batch_bytes;
   return SME_OK;
 }
@@ -215,6 +216,7 @@ int sme_get_tuning(const sme_ctx* ctx, sme_tuning* t) {
   *t = sme_tuning{};
   t->lane_path_max = ctx->impl.tuning.lane_path_max;
   t->item_grid = ctx->impl.tuning.item_grid;
+  t->super_chunks = ctx->impl.tuning.super_chunks;
   t->batch_bytes = ctx->impl.tuning.batch_bytes;
   return SME_OK;
 }
@@ -271,7 +273,7 @@ int sme_autotune(sme_ctx* ctx, const sme_graph* g, const sme_plan* const* plans,
   int n = 0;
   auto record = [&](const sme::Context::Tuning& t, double ms) {
     if (trials && n < cap) {
-      trials[n].tuning = sme_tuning{t.lane_path_max, t.item_grid, 0, t.batch_bytes};
+      trials[n].tuning = sme_tuning{t.lane_path_max, t.item_grid, t.super_chunks, t.batch_bytes};
       trials[n].ms = ms;
     }
     ++n;
@@ -292,19 +294,21 @@ int sme_autotune(sme_ctx* ctx, const sme_graph* g, const sme_plan* const* plans,
   }
   record(cur, cur_ms);
   const uint64_t lane_opts[] = {0, 8, 32, 128, 512};
+  const uint32_t super_opts[] = {0, 2, 4, 8, 16};
   const uint32_t grid_opts[] = {0, 8192, 4096, 2048, 1024};
-  const uint64_t batch_opts[] = {0, 4ull << 30, 1ull << 30};
-  for (int dim = 0; dim < 3; ++dim) {
-    const int nopt = dim == 0 ? 5 : dim == 1 ? 5 : 3;
+  const uint64_t batch_opts[] = {0, 4ull << 30, 1ull << 30, 256ull << 20, 64ull << 20};
+  for (int dim = 0; dim < 4; ++dim) {
+    const int nopt = 5;
     sme::Context::Tuning dim_best = cur;
     double dim_ms = cur_ms;
     for (int k = 0; k < nopt; ++k) {
       sme::Context::Tuning t = cur;
       if (dim == 0) t.lane_path_max = lane_opts[k];
-      if (dim == 1) t.item_grid = grid_opts[k];
-      if (dim == 2) t.batch_bytes = batch_opts[k];
+      if (dim == 1) t.super_chunks = super_opts[k];
+      if (dim == 2) t.item_grid = grid_opts[k];
+      if (dim == 3) t.batch_bytes = batch_opts[k];
       if (t.lane_path_max == cur.lane_path_max && t.item_grid == cur.item_grid &&
-          t.batch_bytes == cur.batch_bytes)
+          t.super_chunks == cur.super_chunks && t.batch_bytes == cur.batch_bytes)
         continue;
       double t_ms = 0;
       rc = measure(t, &t_ms);
@@ -322,7 +326,7 @@ int sme_autotune(sme_ctx* ctx, const sme_graph* g, const sme_plan* const* plans,
     cur_ms = dim_ms;
   }
   c.tuning = cur;
-  *best = sme_tuning{cur.lane_path_max, cur.item_grid, 0, cur.batch_bytes};
+  *best = sme_tuning{cur.lane_path_max, cur.item_grid, cur.super_chunks, cur.batch_bytes};
   if (best_ms) *best_ms = cur_ms;
   if (ntrials) *ntrials = n;
   return SME_OK;

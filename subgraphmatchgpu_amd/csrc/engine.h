@@ -56,10 +56,24 @@ struct RunStats {
 class Context {
  public:
   ~Context();
+  // Own pool of budget_bytes.
   Status Open(int device, uint64_t budget_bytes, std::string* err);
+  // Allocates from `shared` (several contexts on one device, e.g. the slots
+  // of a server); `shared` must outlive the context.
+  Status OpenShared(int device, DevicePool* shared, std::string* err);
   int device() const { return device_; }
   hipStream_t stream() const { return stream_; }
-  DevicePool& pool() { return pool_; }
+  DevicePool& pool() { return *pool_; }
+
+  // Allocation accounting of this context's runs.  With `limit` set (a
+  // reservation in a shared pool) the batch planner sees at most
+  // limit - held bytes.
+  void* Alloc(size_t bytes);
+  void Free(void* p, size_t bytes);
+  uint64_t RunAvailable() const;
+  uint64_t limit = 0;
+  uint64_t held = 0;
+  uint64_t held_peak = 0;
   uint64_t* pinned() { return pinned_; }
 
   // hub adjacency bitmaps for graphs uploaded from now on: vertices of
@@ -74,11 +88,14 @@ class Context {
   //                  paths go through the 64-items-per-wave schedule (0: all
   //                  paths use the item schedule)
   //   item_grid      workgroup cap of the item kernels (0: default)
+  //   super_chunks   chunks per wave in the verify kernels (single-path
+  //                  runs take the staged fast path; 0 or 1: off)
   //   batch_bytes    cap on the device bytes of one batch (0: the level's
   //                  share of the pool)
   struct Tuning {
     uint64_t lane_path_max = 0;
     uint32_t item_grid = 0;
+    uint32_t super_chunks = 0;
     uint64_t batch_bytes = 0;
   } tuning;
 
@@ -103,7 +120,8 @@ class Context {
  private:
   int device_ = -1;
   hipStream_t stream_ = nullptr;
-  DevicePool pool_;
+  DevicePool own_pool_;
+  DevicePool* pool_ = &own_pool_;
   uint64_t* pinned_ = nullptr;
 };
 

@@ -35,12 +35,23 @@ Context::~Context() {
     (void)hipEventDestroy(m.stop);
   }
   for (auto e : spare_events) (void)hipEventDestroy(e);
-  pool_.Release();
+  own_pool_.Release();
   if (pinned_) (void)hipHostFree(pinned_);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
 Status Context::Open(int device, uint64_t budget_bytes, std::string* err) {
+  const Status s = OpenShared(device, nullptr, err);
+  if (s != kOk) return s;
+  if (!own_pool_.Init(budget_bytes)) {
+    *err = "cannot reserve the device pool of " + std::to_string(budget_bytes) + " bytes";
+    return kErrOom;
+  }
+  return kOk;
+}
+
+Status Context::OpenShared(int device, DevicePool* shared, std::string* err) {
+  pool_ = shared ? shared : &own_pool_;
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
     (void)hipGetLastError();
@@ -65,11 +76,28 @@ Status Context::Open(int device, uint64_t budget_bytes, std::string* err) {
     *err = "hipHostMalloc failed";
     return kErrHip;
   }
-  if (!pool_.Init(budget_bytes)) {
-    *err = "cannot reserve the device pool of " + std::to_string(budget_bytes) + " bytes";
-    return kErrOom;
-  }
   return kOk;
+}
+
+void* Context::Alloc(size_t bytes) {
+  void* p = pool_->Alloc(bytes, stream_);
+  if (p) {
+    held += DevicePool::Rounded(bytes);
+    if (held > held_peak) held_peak = held;
+  }
+  return p;
+}
+
+void Context::Free(void* p, size_t bytes) {
+  if (!p) return;
+  pool_->Free(p, stream_);
+  held -= DevicePool::Rounded(bytes);
+}
+
+uint64_t Context::RunAvailable() const {
+  uint64_t avail = std::min<uint64_t>(pool_->Available(), pool_->LargestFree());
+  if (limit) avail = std::min<uint64_t>(avail, limit > held ? limit - held : 0);
+  return avail;
 }
 
 hipEvent_t Context::TakeEvent() {
@@ -180,16 +208,16 @@ constexpr uint64_t kMaxBatchPaths = 1ull << 31;
 template <typename T>
 class Buf {
  public:
-  Buf(DevicePool* pool, uint64_t n) : pool_(pool), n_(n) {
+  Buf(Context* ctx, uint64_t n) : ctx_(ctx), n_(n) {
     if (n == 0) return;
-    p_ = static_cast<T*>(pool->Alloc(n * sizeof(T)));
+    p_ = static_cast<T*>(ctx->Alloc(n * sizeof(T)));
     if (!p_)
       Fail(kErrOom, "device pool exhausted: need " + std::to_string(n * sizeof(T)) + " bytes, " +
-                        std::to_string(pool->Available()) + " available");
+                        std::to_string(ctx->pool().Available()) + " available");
   }
   ~Buf() { Reset(); }
   void Reset() {
-    if (p_) pool_->Free(p_);
+    if (p_) ctx_->Free(p_, n_ * sizeof(T));
     p_ = nullptr;
   }
   Buf(const Buf&) = delete;
@@ -198,7 +226,7 @@ class Buf {
   uint64_t size() const { return n_; }
 
  private:
-  DevicePool* pool_;
+  Context* ctx_;
   uint64_t n_;
   T* p_ = nullptr;
 };
@@ -249,7 +277,7 @@ Frame SliceFrame(const Frame& f, uint64_t l, uint64_t r) {
 class Runner {
  public:
   Runner(Context* ctx, const DevGraph* g, const EnginePlan* plan)
-      : ctx_(ctx), g_(g), plan_(plan), st_(ctx->stream()), pool_(&ctx->pool()) {}
+      : ctx_(ctx), g_(g), plan_(plan), st_(ctx->stream()), pool_(ctx) {}
 
   // Per-bucket counts (device arrays owned by the caller).
   void SetBuckets(const uint32_t* dev_bounds, uint64_t nbuckets, unsigned long long* dev_counts) {
@@ -421,7 +449,7 @@ class Runner {
   }
 
   uint64_t Budget(int L) const {
-    const uint64_t avail = std::min<uint64_t>(pool_->Available(), pool_->LargestFree());
+    const uint64_t avail = ctx_->RunAvailable();
     uint64_t per = static_cast<uint64_t>(0.9 * static_cast<double>(avail) /
                                          static_cast<double>(plan_->nops - L));
     if (ctx_->tuning.batch_bytes) per = std::min<uint64_t>(per, ctx_->tuning.batch_bytes);
@@ -593,6 +621,7 @@ class Runner {
     it.hub_words = g_->hub_words;
     it.cols = g_->cols;
     it.cs = SliceCols(f.cs, l);
+    it.super_chunks = ctx_->tuning.super_chunks;
     return it;
   }
 
@@ -1095,7 +1124,7 @@ class Runner {
   const DevGraph* g_;
   const EnginePlan* plan_;
   hipStream_t st_;
-  DevicePool* pool_;
+  Context* pool_;  // allocations of this run go through the context
   unsigned long long* acc_ = nullptr;
   int lane_lists_[kMaxOps];  // lists per item of lane-counted levels
   uint64_t host_level_[kMaxOps];
@@ -1130,12 +1159,12 @@ Status RunPlan(Context* ctx, const DevGraph* g, const EnginePlan* plan, uint32_t
   const auto t0 = std::chrono::steady_clock::now();
   try {
     SME_HIP(hipSetDevice(ctx->device()));
-    ctx->pool().ResetHighWater();
+    ctx->held_peak = ctx->held;
     Runner r(ctx, g, plan);
     RunStats st;
     r.Run(v_begin, v_end, count, level_counts, &st);
     if (ctx->profiling) ctx->ResolveMarks();
-    st.peak_pool_bytes = ctx->pool().HighWater();
+    st.peak_pool_bytes = ctx->held_peak;
     st.elapsed_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (stats) *stats = st;
@@ -1177,9 +1206,9 @@ Status RunPlanBuckets(Context* ctx, const DevGraph* g, const EnginePlan* plan, c
   const auto t0 = std::chrono::steady_clock::now();
   try {
     SME_HIP(hipSetDevice(ctx->device()));
-    ctx->pool().ResetHighWater();
-    Buf<uint32_t> dbounds(&ctx->pool(), nbuckets + 1);
-    Buf<unsigned long long> dcounts(&ctx->pool(), nbuckets);
+    ctx->held_peak = ctx->held;
+    Buf<uint32_t> dbounds(ctx, nbuckets + 1);
+    Buf<unsigned long long> dcounts(ctx, nbuckets);
     SME_HIP(hipMemcpyAsync(dbounds.get(), bounds, (nbuckets + 1) * sizeof(uint32_t), hipMemcpyHostToDevice,
                            ctx->stream()));
     LaunchFill64(reinterpret_cast<uint64_t*>(dcounts.get()), nbuckets, 0, ctx->stream());
@@ -1191,7 +1220,7 @@ Status RunPlanBuckets(Context* ctx, const DevGraph* g, const EnginePlan* plan, c
                            ctx->stream()));
     SME_HIP(hipStreamSynchronize(ctx->stream()));
     if (ctx->profiling) ctx->ResolveMarks();
-    st.peak_pool_bytes = ctx->pool().HighWater();
+    st.peak_pool_bytes = ctx->held_peak;
     st.elapsed_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (stats) *stats = st;

@@ -127,6 +127,10 @@ struct ItemArgs {
   unsigned long long* acc;        // total count
   unsigned long long* acc_level;  // this level's contribution
   BucketRef bk;                   // counting modes only
+  // verify kernels: a wave takes super_chunks consecutive chunks; when they
+  // all lie in one segment the per-segment data is loaded once and list
+  // windows are staged in LDS (0 or 1: one chunk at a time)
+  uint32_t super_chunks;
 };
 
 // COMPUTE_COUNT set-up with lane-per-path counting of short paths: paths

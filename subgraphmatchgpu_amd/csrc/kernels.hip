@@ -13,6 +13,7 @@ constexpr int kThreads = 256;
 constexpr int kWave = 64;
 constexpr int kWavesPerBlock = kThreads / kWave;
 constexpr uint64_t kMaxGrid = 16384;
+constexpr uint32_t kStageCap = 512;  // LDS entries per wave for a staged list window
 
 using u64 = unsigned long long;
 
@@ -331,61 +332,236 @@ __device__ __forceinline__ const uint32_t* ItemSrc(const ItemArgs& a, uint64_t s
   return (a.has_reuse && a.piv[s] == a.nb) ? a.reuse.base : a.src;
 }
 
+// One 64-item chunk through the general per-lane path (items of several
+// segments).  Returns the lane's count contribution (sum modes) or writes
+// the chunk's mask and popcount (mask mode).
 template <int kMode>
-__global__ __launch_bounds__(kThreads) void VerifyKernel(ItemArgs a) {
-  const int lane = threadIdx.x & (kWave - 1);
-  const uint64_t wave = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) / kWave;
-  const uint64_t nwaves = GridStride() / kWave;
-  u64 local = 0;
-  for (uint64_t c = wave; c < a.nchunks; c += nwaves) {
-    const uint64_t t = c * kWave + lane;
-    bool ok = t < a.nitems;
-    uint64_t s = 0;
-    u64 val = 0;
-    if (ok) {
-      s = SegmentOf(a, c, t);
-      const uint64_t r = a.offs[0] + t - a.offs[s];
-      if (kMode == kModeSumTwo) {
-        // GeneralCountTwoComputedVertices, LIGHTCount.cuh:22-116: walk the
-        // smaller set, binary-search the larger one.
-        const bool sw = a.swapped[s] != 0;
-        const CandRef& sm = sw ? a.second : a.pair;
-        const CandRef& bg = sw ? a.pair : a.second;
-        const uint32_t x = sm.base[a.pstart[s] + r];
-        const uint64_t bs = SegIndex(bg, s);
-        const uint32_t* arr = bg.base + bg.beg[bs];
-        const uint64_t n = bg.len[bs];
-        const uint64_t pos = LowerBound(arr, 0, n, x);
-        const bool eq = pos < n && arr[pos] == x;
-        if (a.pair_op == 0)
-          val = sw ? pos : n - pos - (eq ? 1 : 0);
-        else if (a.pair_op == 1)
-          val = sw ? n - pos - (eq ? 1 : 0) : pos;
-        else
-          val = n - (eq ? 1 : 0);
-      } else {
-        const uint32_t x = ItemSrc(a, s)[a.pstart[s] + r];
-        ok = ItemValid(a, s, x);
-        if (kMode == kModeSum) {
-          val = ok ? 1 : 0;
-        } else if (kMode == kModeSumPair && ok) {
-          const uint64_t ps = SegIndex(a.pair, s);
-          val = CountWithComputed(a.pair.base + a.pair.beg[ps], a.pair.len[ps], x, a.pair_op);
-        }
+__device__ __forceinline__ u64 GenericChunk(const ItemArgs& a, uint64_t c, int lane) {
+  const uint64_t t = c * kWave + lane;
+  bool ok = t < a.nitems;
+  uint64_t s = 0;
+  u64 val = 0;
+  if (ok) {
+    s = SegmentOf(a, c, t);
+    const uint64_t r = a.offs[0] + t - a.offs[s];
+    if (kMode == kModeSumTwo) {
+      // GeneralCountTwoComputedVertices, LIGHTCount.cuh:22-116: walk the
+      // smaller set, binary-search the larger one.
+      const bool sw = a.swapped[s] != 0;
+      const CandRef& sm = sw ? a.second : a.pair;
+      const CandRef& bg = sw ? a.pair : a.second;
+      const uint32_t x = sm.base[a.pstart[s] + r];
+      const uint64_t bs = SegIndex(bg, s);
+      const uint32_t* arr = bg.base + bg.beg[bs];
+      const uint64_t n = bg.len[bs];
+      const uint64_t pos = LowerBound(arr, 0, n, x);
+      const bool eq = pos < n && arr[pos] == x;
+      if (a.pair_op == 0)
+        val = sw ? pos : n - pos - (eq ? 1 : 0);
+      else if (a.pair_op == 1)
+        val = sw ? n - pos - (eq ? 1 : 0) : pos;
+      else
+        val = n - (eq ? 1 : 0);
+    } else {
+      const uint32_t x = ItemSrc(a, s)[a.pstart[s] + r];
+      ok = ItemValid(a, s, x);
+      if (kMode == kModeSum) {
+        val = ok ? 1 : 0;
+      } else if (kMode == kModeSumPair && ok) {
+        const uint64_t ps = SegIndex(a.pair, s);
+        val = CountWithComputed(a.pair.base + a.pair.beg[ps], a.pair.len[ps], x, a.pair_op);
       }
     }
+  }
+  if (kMode == kModeMask) {
+    const u64 m = __ballot(ok);
+    if (lane == 0) {
+      a.mask[c] = m;
+      a.ccount[c] = static_cast<uint32_t>(__popcll(m));
+    }
+  } else if (a.bk.bounds) {
+    const u64 key = t < a.nitems ? BucketOf(a.bk, Gather(a.cs, a.bk.start_v, s)) : kNoBucket;
+    BucketAdd(a.bk, key, val);
+  }
+  return val;
+}
+
+// Wave-level ordering point for LDS staging: every lane's earlier LDS
+// accesses complete before any lane's later ones.
+__device__ __forceinline__ void WaveSync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ uint64_t Uniform64(uint64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v));
+  const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v >> 32));
+  return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
+__device__ __forceinline__ bool ContainsLds(const uint32_t* a, uint32_t n, uint32_t key) {
+  uint32_t lo = 0, hi = n;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (a[mid] < key)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo < n && a[lo] == key;
+}
+
+// A sorted list to test against: a hub bitmap row, or entries a[lo, hi).
+struct ListRef {
+  const uint32_t* a;
+  const uint32_t* hrow;
+  uint64_t lo, hi;
+};
+
+// Lists other than the pivot of segment s (at most two are kept; callers
+// check OtherLists(a) <= 2 first).
+__device__ __forceinline__ int OtherLists(const ItemArgs& a) {
+  return a.nb + a.has_reuse > 1 ? a.nb + a.has_reuse - 1 : 0;
+}
+
+// The k-th list (k = 0, 1) other than the pivot pv of segment s.
+__device__ __forceinline__ ListRef OtherList(const ItemArgs& a, uint64_t s, uint8_t pv, int k) {
+  // adjacency lists in bconn order skipping the pivot, then the reused segment
+  int j = k + (pv <= k ? 1 : 0);
+  if (pv == a.nb) j = k;  // the pivot is the reused segment
+  if (j < a.nb) {
+    const uint32_t v = Gather(a.cs, a.bconn[j], s);
+    const uint32_t h = a.hub_id ? a.hub_id[v] : kNoHub;
+    return ListRef{a.cols, h != kNoHub ? a.hub_bits + static_cast<uint64_t>(h) * a.hub_words : nullptr,
+                   a.rp[v], a.rp[v + 1]};
+  }
+  const uint64_t seg = SegIndex(a.reuse, s);
+  const uint64_t b = a.reuse.beg[seg];
+  return ListRef{a.reuse.base, nullptr, b, b + a.reuse.len[seg]};
+}
+
+__device__ __forceinline__ void CollectLists(const ItemArgs& a, uint64_t s, uint8_t pv, ListRef* l0,
+                                             ListRef* l1) {
+  const int n = OtherLists(a);
+  if (n > 0) *l0 = OtherList(a, s, pv, 0);
+  if (n > 1) *l1 = OtherList(a, s, pv, 1);
+}
+
+__device__ __forceinline__ void NarrowToWindow(ListRef* r, uint32_t x_first, uint32_t x_last) {
+  const uint64_t lo = LowerBound(r->a, r->lo, r->hi, x_first);
+  uint64_t hi = LowerBound(r->a, lo, r->hi, x_last);
+  if (hi < r->hi && r->a[hi] == x_last) ++hi;
+  r->lo = lo;
+  r->hi = hi;
+}
+
+__device__ __forceinline__ bool Member(const ListRef& r, bool staged, const uint32_t* stage,
+                                       uint32_t nstage, uint32_t x) {
+  if (r.hrow) return (r.hrow[x >> 5] >> (x & 31)) & 1u;
+  if (staged) return ContainsLds(stage, nstage, x);
+  return Contains(r.a, r.lo, r.hi, x);
+}
+
+// Chunks [c0, c1) whose items all belong to segment s (at most two other
+// lists and two non-equality conditions).  Everything per segment is loaded
+// once; the first non-hub list of at most kStageCap entries is staged in LDS
+// whole (independent loads: one memory round trip), turning its membership
+// tests into LDS binary searches; a longer list is cut to the value window
+// [x_first, x_last] of the run, the items being one sorted run of the pivot.
+template <int kMode>
+__device__ u64 FastSegment(const ItemArgs& a, uint64_t s, uint64_t c0, uint64_t c1,
+                           uint32_t* stage, int lane) {
+  s = Uniform64(s);
+  const uint8_t pv = (a.nb + a.has_reuse > 1) ? a.piv[s] : 0;
+  const uint32_t* src = (a.has_reuse && pv == a.nb) ? a.reuse.base : a.src;
+  const uint64_t t0 = c0 * kWave;
+  const uint64_t t1 = (c1 * kWave < a.nitems) ? c1 * kWave : a.nitems;
+  const uint64_t base = a.pstart[s] + (a.offs[0] + t0 - a.offs[s]);
+  const int nl = OtherLists(a);
+  ListRef l0{nullptr, nullptr, 0, 0}, l1{nullptr, nullptr, 0, 0};
+  CollectLists(a, s, pv, &l0, &l1);
+  // staged list: 0 = l0, 1 = l1, -1 none
+  int staged = -1;
+  if (nl > 0 && !l0.hrow && l0.hi - l0.lo <= kStageCap)
+    staged = 0;
+  else if (nl > 1 && !l1.hrow && l1.hi - l1.lo <= kStageCap)
+    staged = 1;
+  uint32_t nstage = 0;
+  if (staged >= 0) {
+    const uint32_t* sa = staged == 0 ? l0.a : l1.a;
+    const uint64_t slo = staged == 0 ? l0.lo : l1.lo;
+    nstage = static_cast<uint32_t>((staged == 0 ? l0.hi : l1.hi) - slo);
+    WaveSync();  // earlier readers of the stage are done
+    for (uint32_t i = lane; i < nstage; i += kWave) stage[i] = sa[slo + i];
+    WaveSync();
+  }
+  const uint32_t x_first = src[base], x_last = src[base + (t1 - t0) - 1];
+  if (nl > 0 && !l0.hrow && staged != 0) NarrowToWindow(&l0, x_first, x_last);
+  if (nl > 1 && !l1.hrow && staged != 1) NarrowToWindow(&l1, x_first, x_last);
+  const uint32_t ne0 = a.nne > 0 ? Gather(a.cs, a.ne_v[0], s) : 0u;
+  const uint32_t ne1 = a.nne > 1 ? Gather(a.cs, a.ne_v[1], s) : 0u;
+  const uint32_t* parr = nullptr;
+  uint64_t plen = 0;
+  if (kMode == kModeSumPair) {
+    const uint64_t ps = SegIndex(a.pair, s);
+    parr = a.pair.base + a.pair.beg[ps];
+    plen = a.pair.len[ps];
+  }
+  u64 local = 0;
+  for (uint64_t c = c0; c < c1; ++c) {
+    const uint64_t t = c * kWave + lane;
+    bool ok = t < t1;
+    uint32_t x = 0;
+    if (ok) x = src[base + (t - t0)];
+    if (nl > 0 && ok) ok = Member(l0, staged == 0, stage, nstage, x);
+    if (nl > 1 && ok) ok = Member(l1, staged == 1, stage, nstage, x);
+    if (a.nne > 0) ok = ok && x != ne0;
+    if (a.nne > 1) ok = ok && x != ne1;
     if (kMode == kModeMask) {
       const u64 m = __ballot(ok);
       if (lane == 0) {
         a.mask[c] = m;
         a.ccount[c] = static_cast<uint32_t>(__popcll(m));
       }
-    } else {
-      local += val;
-      if (a.bk.bounds) {
-        const u64 key = t < a.nitems ? BucketOf(a.bk, Gather(a.cs, a.bk.start_v, s)) : kNoBucket;
-        BucketAdd(a.bk, key, val);
+    } else if (kMode == kModeSum) {
+      local += ok ? 1 : 0;
+    } else if (ok) {
+      local += CountWithComputed(parr, plen, x, a.pair_op);
+    }
+  }
+  return local;
+}
+
+template <int kMode>
+__global__ __launch_bounds__(kThreads) void VerifyKernel(ItemArgs a) {
+  __shared__ uint32_t stage_all[kWavesPerBlock][kStageCap];
+  const int lane = threadIdx.x & (kWave - 1);
+  uint32_t* stage = stage_all[threadIdx.x / kWave];
+  const uint64_t wave = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) / kWave;
+  const uint64_t nwaves = GridStride() / kWave;
+  const uint64_t M = a.super_chunks > 1 ? a.super_chunks : 1;
+  const uint64_t nsuper = (a.nchunks + M - 1) / M;
+  u64 local = 0;
+  for (uint64_t sc = wave; sc < nsuper; sc += nwaves) {
+    const uint64_t c0 = sc * M;
+    const uint64_t c1 = c0 + M < a.nchunks ? c0 + M : a.nchunks;
+    bool fast = false;
+    uint64_t s = 0;
+    if (kMode != kModeSumTwo && M > 1 && OtherLists(a) <= 2 && a.nne <= 2) {
+      const uint64_t t_last = (c1 * kWave < a.nitems ? c1 * kWave : a.nitems) - 1;
+      s = a.chunk_seg[c0];
+      fast = SegmentOf(a, c1 - 1, t_last) == s;
+    }
+    if (fast) {
+      u64 seg = FastSegment<kMode>(a, s, c0, c1, stage, lane);
+      local += seg;
+      if (kMode != kModeMask && a.bk.bounds) {
+        seg = WaveSum(seg);
+        if (lane == 0 && seg) atomicAdd(a.bk.counts + BucketOf(a.bk, Gather(a.cs, a.bk.start_v, s)), seg);
       }
+    } else {
+      for (uint64_t c = c0; c < c1; ++c) local += GenericChunk<kMode>(a, c, lane);
     }
   }
   if (kMode != kModeMask) {
@@ -395,6 +571,74 @@ __global__ __launch_bounds__(kThreads) void VerifyKernel(ItemArgs a) {
       atomicAdd(a.acc_level, local);
     }
   }
+}
+
+// Membership of x in the sorted list a[*pos, end) for increasing keys: the
+// cursor only moves forward, and one round of 8 independent probes past it
+// usually lands on the key's position; otherwise a binary search over the
+// rest finds it.  Leaves *pos at the first entry >= x.
+__device__ __forceinline__ bool Gallop(const uint32_t* a, uint64_t* pos, uint64_t end, uint32_t x) {
+  const uint64_t p = *pos;
+  uint32_t lt = 0;
+  bool eq = false;
+  const uint32_t* ap = a + p;
+  const uint64_t room = end - p;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const uint32_t y = static_cast<uint64_t>(k) < room ? ap[k] : 0xffffffffu;
+    lt += y < x ? 1u : 0u;
+    eq = eq || (y == x && static_cast<uint64_t>(k) < room);
+  }
+  if (lt < 8) {
+    *pos = p + lt;
+    return eq;
+  }
+  const uint64_t q = LowerBound(a, p + 8, end, x);
+  *pos = q;
+  return q < end && a[q] == x;
+}
+
+// Count of path p's pivot items [b, e) that pass every other list and the
+// non-equality conditions (with the pair arithmetic for COMPUTE_COUNT with
+// one other unmaterialized vertex).  One lane walks the path's items in
+// order, galloping through each other list (at most two; with more, each
+// item is binary-searched).
+__device__ u64 LanePathCount(const PrepCountArgs& a, uint64_t p, uint8_t pv, uint64_t b,
+                             uint64_t e) {
+  const ItemArgs& it = a.it;
+  const uint32_t* src = (it.has_reuse && pv == it.nb) ? it.reuse.base : it.cols;
+  const uint32_t* parr = nullptr;
+  uint64_t plen = 0;
+  if (a.pair) {
+    const uint64_t ps = SegIndex(it.pair, p);
+    parr = it.pair.base + it.pair.beg[ps];
+    plen = it.pair.len[ps];
+  }
+  u64 val = 0;
+  const int nl = OtherLists(it);
+  if (nl > 2 || it.nne > 2) {
+    for (uint64_t i = b; i < e; ++i) {
+      const uint32_t x = src[i];
+      if (ItemValidPv(it, p, pv, x)) val += a.pair ? CountWithComputed(parr, plen, x, it.pair_op) : 1ull;
+    }
+    return val;
+  }
+  ListRef l0{nullptr, nullptr, 0, 0}, l1{nullptr, nullptr, 0, 0};
+  CollectLists(it, p, pv, &l0, &l1);
+  const uint32_t ne0 = it.nne > 0 ? Gather(it.cs, it.ne_v[0], p) : 0u;
+  const uint32_t ne1 = it.nne > 1 ? Gather(it.cs, it.ne_v[1], p) : 0u;
+  for (uint64_t i = b; i < e; ++i) {
+    const uint32_t x = src[i];
+    bool ok = true;
+    if (nl > 0)
+      ok = l0.hrow ? ((l0.hrow[x >> 5] >> (x & 31)) & 1u) != 0 : Gallop(l0.a, &l0.lo, l0.hi, x);
+    if (nl > 1 && ok)
+      ok = l1.hrow ? ((l1.hrow[x >> 5] >> (x & 31)) & 1u) != 0 : Gallop(l1.a, &l1.lo, l1.hi, x);
+    if (it.nne > 0) ok = ok && x != ne0;
+    if (it.nne > 1) ok = ok && x != ne1;
+    if (ok) val += a.pair ? CountWithComputed(parr, plen, x, it.pair_op) : 1ull;
+  }
+  return val;
 }
 
 // COMPUTE_COUNT set-up that also counts short paths: a path whose pivot
@@ -415,19 +659,7 @@ __global__ __launch_bounds__(kThreads) void PrepCountKernel(PrepCountArgs a) {
       PrepPath(a.pa, p, &b, &e, &pv);
       uint64_t n = e - b;
       if (n <= a.lane_max) {
-        const uint32_t* src = (a.it.has_reuse && pv == a.it.nb) ? a.it.reuse.base : a.it.cols;
-        uint64_t ps = 0, plen = 0;
-        const uint32_t* parr = nullptr;
-        if (a.pair) {
-          ps = SegIndex(a.it.pair, p);
-          parr = a.it.pair.base + a.it.pair.beg[ps];
-          plen = a.it.pair.len[ps];
-        }
-        for (uint64_t i = b; i < e; ++i) {
-          const uint32_t x = src[i];
-          if (!ItemValidPv(a.it, p, pv, x)) continue;
-          val += a.pair ? CountWithComputed(parr, plen, x, a.it.pair_op) : 1ull;
-        }
+        if (n) val = LanePathCount(a, p, pv, b, e);
         nlane += n;
         n = 0;
       }
@@ -705,7 +937,8 @@ void LaunchChunkPartition(const uint64_t* offs, uint64_t nseg, uint64_t nchunks,
 
 void LaunchVerify(const ItemArgs& a, int mode, uint32_t max_grid, hipStream_t stream) {
   if (a.nchunks == 0) return;
-  const uint64_t grid = GridFor(a.nchunks, kWavesPerBlock, max_grid);
+  const uint64_t M = a.super_chunks > 1 ? a.super_chunks : 1;
+  const uint64_t grid = GridFor((a.nchunks + M - 1) / M, kWavesPerBlock, max_grid);
   switch (mode) {
     case kModeMask: hipLaunchKernelGGL(VerifyKernel<kModeMask>, dim3(grid), dim3(kThreads), 0, stream, a); break;
     case kModeSum: hipLaunchKernelGGL(VerifyKernel<kModeSum>, dim3(grid), dim3(kThreads), 0, stream, a); break;

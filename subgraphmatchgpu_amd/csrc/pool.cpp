@@ -9,6 +9,7 @@ DevicePool::~DevicePool() { Release(); }
 
 bool DevicePool::Init(size_t bytes) {
   Release();
+  std::lock_guard<std::mutex> lk(mu_);
   bytes = (bytes / kAlign) * kAlign;
   if (bytes == 0) return true;
   void* p = nullptr;
@@ -25,6 +26,10 @@ bool DevicePool::Init(size_t bytes) {
 }
 
 void DevicePool::Release() {
+  std::lock_guard<std::mutex> lk(mu_);
+  ReclaimLocked(nullptr, true);
+  for (hipEvent_t e : spare_events_) (void)hipEventDestroy(e);
+  spare_events_.clear();
   if (base_) (void)hipFree(base_);
   base_ = nullptr;
   capacity_ = used_ = high_water_ = 0;
@@ -43,11 +48,11 @@ void DevicePool::EraseFree(size_t off, size_t len) {
   free_by_size_.erase({len, off});
 }
 
-size_t DevicePool::LargestFree() const {
+size_t DevicePool::LargestFreeLocked() const {
   return free_by_size_.empty() ? 0 : free_by_size_.rbegin()->first;
 }
 
-void* DevicePool::Alloc(size_t bytes) {
+void* DevicePool::AllocLocked(size_t bytes) {
   if (bytes == 0 || !base_) return nullptr;
   const size_t len = (bytes + kAlign - 1) / kAlign * kAlign;
   auto it = free_by_size_.lower_bound({len, 0});  // best fit
@@ -61,7 +66,7 @@ void* DevicePool::Alloc(size_t bytes) {
   return base_ + blk_off;
 }
 
-void DevicePool::Free(void* p) {
+void DevicePool::FreeLocked(void* p) {
   if (!p) return;
   size_t off = static_cast<size_t>(static_cast<char*>(p) - base_);
   auto lv = live_.find(off);

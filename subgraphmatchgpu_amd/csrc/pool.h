@@ -11,13 +11,24 @@
 // Exceeding the arena is an error (kErrOom), never a silent fallback to
 // managed memory (the reference spills to cudaMallocManaged,
 // CudaContext.cuh:152-163).
+//
+// Shared mode (several contexts, each with its own stream, carving one
+// arena, as the slots of a server do): a block freed by one stream is
+// handed to another stream only after an event recorded at the free has
+// completed, so a kernel of the new owner never overwrites data a kernel of
+// the old owner may still read.  The freeing stream itself may reuse the
+// block at once.  All public members lock the pool.
 #pragma once
+
+#include <hip/hip_runtime.h>
 
 #include <cstddef>
 #include <cstdint>
 #include <map>
+#include <mutex>
 #include <set>
 #include <utility>
+#include <vector>
 
 namespace sme {
 
@@ -33,21 +44,40 @@ class DevicePool {
   void Release();
 
   // nullptr when the request does not fit; `bytes` == 0 returns nullptr.
-  void* Alloc(size_t bytes);
-  void Free(void* p);
+  // `stream` is the stream whose kernels use the block (shared mode).
+  void* Alloc(size_t bytes, hipStream_t stream = nullptr);
+  void Free(void* p, hipStream_t stream = nullptr);
+  void SetShared(bool on);
 
   size_t Capacity() const { return capacity_; }
-  size_t Used() const { return used_; }
-  size_t Available() const { return capacity_ - used_; }
-  size_t HighWater() const { return high_water_; }
+  size_t Used() const;
+  size_t Available() const;
+  size_t HighWater() const;
   size_t LargestFree() const;
-  void ResetHighWater() { high_water_ = used_; }
+  void ResetHighWater();
 
   static constexpr size_t kAlign = 256;
+  static size_t Rounded(size_t bytes) { return (bytes + kAlign - 1) / kAlign * kAlign; }
 
  private:
+  void* AllocLocked(size_t bytes);
+  void FreeLocked(void* p);
+  size_t LargestFreeLocked() const;
+  // Returns deferred blocks whose free event completed (or that `stream`
+  // freed itself); with `wait`, waits for all of them.
+  void ReclaimLocked(hipStream_t stream, bool wait);
   void InsertFree(size_t off, size_t len);
   void EraseFree(size_t off, size_t len);
+
+  struct Deferred {
+    void* p;
+    hipEvent_t done;
+    hipStream_t stream;
+  };
+  mutable std::mutex mu_;
+  bool shared_ = false;
+  std::vector<Deferred> deferred_;
+  std::vector<hipEvent_t> spare_events_;
 
   char* base_ = nullptr;
   size_t capacity_ = 0;

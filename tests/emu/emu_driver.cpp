@@ -33,13 +33,15 @@ int main(int argc, char** argv) {
     return 1;
   }
   if (argc == 10) sme_set_hub_bitmaps(ctx, std::strtoull(argv[9], nullptr, 10), 1ull << 30);
-  // TUNE="lane_path_max,item_grid,batch_bytes" sets the schedule tunables
+  // TUNE="lane_path_max,item_grid,batch_bytes[,super_chunks]" sets the
+  // schedule tunables
   if (const char* env = std::getenv("TUNE")) {
     sme_tuning t{};
     char* s = nullptr;
     t.lane_path_max = std::strtoull(env, &s, 10);
     if (*s == ',') t.item_grid = static_cast<uint32_t>(std::strtoul(s + 1, &s, 10));
     if (*s == ',') t.batch_bytes = std::strtoull(s + 1, &s, 10);
+    if (*s == ',') t.super_chunks = static_cast<uint32_t>(std::strtoul(s + 1, &s, 10));
     sme_set_tuning(ctx, &t);
   }
   sme_plan* plan = nullptr;

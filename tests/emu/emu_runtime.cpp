@@ -189,6 +189,8 @@ void Launch(dim3 grid, dim3 block, const std::function<void()>& body) {
           } else if (kind == kShflDown) {
             const unsigned src = i + l.arg;
             l.result = src < hi - lo ? vals[src] : vals[i];
+          } else if (kind == kReadLane) {
+            l.result = vals[l.arg < hi - lo ? l.arg : 0];
           } else {
             l.result = i >= l.arg ? vals[i - l.arg] : vals[i];
           }
@@ -277,6 +279,11 @@ hipError_t hipEventDestroy(hipEvent_t e) {
   delete e;
   return hipSuccess;
 }
+hipError_t hipEventCreateWithFlags(hipEvent_t* e, unsigned) { return hipEventCreate(e); }
+// work is synchronous here, so every recorded event has completed
+hipError_t hipEventQuery(hipEvent_t) { return hipSuccess; }
+hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
+
 hipError_t hipEventRecord(hipEvent_t e, hipStream_t) {
   e->t = std::chrono::steady_clock::now();
   return hipSuccess;

@@ -54,6 +54,10 @@ hipError_t hipMemsetAsync(void* d, int v, size_t n, hipStream_t st);
 hipError_t hipEventCreate(hipEvent_t* e);
 hipError_t hipEventDestroy(hipEvent_t e);
 hipError_t hipEventRecord(hipEvent_t e, hipStream_t s);
+hipError_t hipEventCreateWithFlags(hipEvent_t* e, unsigned flags);
+hipError_t hipEventQuery(hipEvent_t e);
+hipError_t hipEventSynchronize(hipEvent_t e);
+#define hipEventDisableTiming 2
 hipError_t hipEventElapsedTime(float* ms, hipEvent_t a, hipEvent_t b);
 
 #define __global__
@@ -69,7 +73,7 @@ extern dim3 g_block_idx, g_block_dim, g_grid_dim;
 void Launch(dim3 grid, dim3 block, const std::function<void()>& body);
 unsigned long long WaveExchange(int kind, unsigned long long v, unsigned arg);
 void BlockBarrier();
-enum { kBallot = 1, kShflXor = 2, kShflUp = 3, kShflDown = 4 };
+enum { kBallot = 1, kShflXor = 2, kShflUp = 3, kShflDown = 4, kReadLane = 5 };
 template <typename T>
 inline T FromBits(unsigned long long b) {
   T t;
@@ -105,6 +109,13 @@ template <typename T>
 inline T __shfl_down(T v, unsigned delta, int width = 64) {
   (void)width;
   return emu::FromBits<T>(emu::WaveExchange(emu::kShflDown, emu::ToBits(v), delta));
+}
+// Wave-level builtins used by the kernels: a wave barrier is a collective
+// scheduling point; readfirstlane returns lane 0's value.
+#define __builtin_amdgcn_fence(order, scope) ((void)0)
+inline void __builtin_amdgcn_wave_barrier() { (void)emu::WaveExchange(emu::kBallot, 0ull, 0); }
+inline unsigned int __builtin_amdgcn_readfirstlane(unsigned int v) {
+  return static_cast<unsigned int>(emu::WaveExchange(emu::kReadLane, v, 0));
 }
 inline unsigned int __popcll(unsigned long long x) { return static_cast<unsigned>(__builtin_popcountll(x)); }
 inline unsigned long long __lanemask_lt() {

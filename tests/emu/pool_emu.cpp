@@ -11,6 +11,7 @@ DevicePool::~DevicePool() { Release(); }
 
 bool DevicePool::Init(size_t bytes) {
   Release();
+  std::lock_guard<std::mutex> lk(mu_);
   capacity_ = (bytes / kAlign) * kAlign;
   used_ = high_water_ = 0;
   base_ = reinterpret_cast<char*>(1);
@@ -18,6 +19,10 @@ bool DevicePool::Init(size_t bytes) {
 }
 
 void DevicePool::Release() {
+  std::lock_guard<std::mutex> lk(mu_);
+  ReclaimLocked(nullptr, true);
+  for (hipEvent_t e : spare_events_) (void)hipEventDestroy(e);
+  spare_events_.clear();
   for (auto& kv : live_) std::free(reinterpret_cast<void*>(kv.first));
   live_.clear();
   free_by_off_.clear();
@@ -29,9 +34,9 @@ void DevicePool::Release() {
 void DevicePool::InsertFree(size_t, size_t) {}
 void DevicePool::EraseFree(size_t, size_t) {}
 
-size_t DevicePool::LargestFree() const { return capacity_ - used_; }
+size_t DevicePool::LargestFreeLocked() const { return capacity_ - used_; }
 
-void* DevicePool::Alloc(size_t bytes) {
+void* DevicePool::AllocLocked(size_t bytes) {
   if (bytes == 0 || !base_) return nullptr;
   const size_t len = (bytes + kAlign - 1) / kAlign * kAlign;
   if (used_ + len > capacity_) return nullptr;
@@ -43,7 +48,7 @@ void* DevicePool::Alloc(size_t bytes) {
   return p;
 }
 
-void DevicePool::Free(void* p) {
+void DevicePool::FreeLocked(void* p) {
   if (!p) return;
   auto it = live_.find(reinterpret_cast<size_t>(p));
   if (it == live_.end()) return;

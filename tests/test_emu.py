@@ -163,3 +163,26 @@ def test_emu_tunings_do_not_change_results(emu, graphs, tune):
         got = [int(x) for x in r.stdout.split("buckets=")[1].split(",")]
         assert got == [oracle.light_count(g.row_ptrs, g.cols, q, 3, v_begin=a, v_end=b)[0]
                        for a, b in zip(bounds[:-1], bounds[1:])], q
+
+
+def test_emu_super_chunk_fast_path(emu, tmp_path):
+    """Runs of chunks inside one path (long candidate lists) take the
+    staged fast path of the verify kernels; counts and levels unchanged,
+    with and without hub bitmaps."""
+    from conftest import clique
+
+    cases = [(sme.gen_rmat(9, 32, seed=5), ((0, "0,0,0,2", None), (0, "0,0,0,4", "16"),
+                                            (2, "0,0,0,4", "16"), (3, "0,0,0,16", None))),
+             (clique(200), ((0, "0,0,0,4", "16"),))]
+    for i, (g, runs) in enumerate(cases):
+        p = str(tmp_path / f"g{i}.bin")
+        sme.write_bin(p, g)
+        for q, tune, hub in runs:
+            want = oracle.light_count(g.row_ptrs, g.cols, q, 3)
+            env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0", TUNE=tune)
+            cmd = [emu, p, "6", str(q), "3", "1", "0", str(g.n), str(1 << 30)] + ([hub] if hub else [])
+            r = subprocess.run(cmd, capture_output=True, text=True, env=env, timeout=600)
+            assert r.returncode == 0, r.stderr[-2000:]
+            parts = r.stdout.split()
+            assert int(parts[0].split("=")[1]) == want[0], (i, q, tune)
+            assert [int(x) for x in parts[1].split("=")[1].split(",")] == want[1], (i, q, tune)

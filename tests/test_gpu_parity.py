@@ -200,7 +200,8 @@ def test_self_loops_handled(dev):
 
 
 @pytest.mark.parametrize("tune", [dict(lane_path_max=8), dict(lane_path_max=1 << 20, item_grid=64),
-                                  dict(item_grid=1, batch_bytes=1 << 16)])
+                                  dict(item_grid=1, batch_bytes=1 << 16), dict(super_chunks=2),
+                                  dict(super_chunks=16, lane_path_max=8, item_grid=512)])
 def test_tunings_do_not_change_results(fixture_graphs, tune):
     g = fixture_graphs["rmat10"]
     d = sme.Device(0, budget_bytes=2 << 30)
@@ -218,6 +219,33 @@ def test_tunings_do_not_change_results(fixture_graphs, tune):
             assert got.tolist() == [oracle.light_count(g.row_ptrs, g.cols, q, 3, v_begin=a, v_end=b)[0]
                                     for a, b in zip(bounds[:-1], bounds[1:])]
     d.close()
+
+
+_ORACLE_CACHE = {}
+
+
+def light_oracle(name, g, q, cl=3):
+    key = (name, q, cl)
+    if key not in _ORACLE_CACHE:
+        _ORACLE_CACHE[key] = oracle.light_count(g.row_ptrs, g.cols, q, cl)
+    return _ORACLE_CACHE[key]
+
+
+@pytest.mark.parametrize("super_chunks", [2, 4, 16])
+def test_super_chunks_with_hubs(fixture_graphs, super_chunks):
+    """Long single-path runs (staged fast path) with and without hub rows."""
+    g = fixture_graphs["rmat12"]
+    for hub in (0, 32):
+        d = sme.Device(0, budget_bytes=2 << 30)
+        d.set_hub_bitmaps(hub, 1 << 30)
+        d.set_tuning(super_chunks=super_chunks)
+        dg = d.upload(g)
+        for q in (0, 1, 2, 3, 4, 7):
+            say(f"case super_chunks={super_chunks} hub={hub} q={q}")
+            want = light_oracle("rmat12", g, q)
+            r = d.count(dg, sme.Plan(q))
+            assert (r.count, r.levels) == want
+        d.close()
 
 
 def test_er_batch_members_one_pass(dev):
