@@ -20,7 +20,16 @@ bucketed pass per pattern (value = member graphs per second), after the
 autotuner has picked the schedule; the same members counted with one engine
 pass per member are timed on a sample for comparison.
 
-Usage: python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--workload rmat20|er-batch]
+Third workload (--workload serve): a resident R-MAT scale-22 graph (edge
+factor 32); a seeded Poisson stream of pattern-count requests (200/s for
+60 s; triangle/4-cycle/diamond/4-clique/house/5-clique with weights
+0.3/0.2/0.2/0.15/0.1/0.05; each restricted to a random start range of 1-100%
+of the vertices) is replayed in real time into the server (continuous
+batching, admission under a 64 GB cap); latency p50/p99 and throughput are
+reported beside the plain one-request-at-a-time path on the same stream,
+and every request's count is checked against the plain path's.
+
+Usage: python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--workload rmat20|er-batch|serve]
 """
 import argparse
 import json
@@ -71,6 +80,35 @@ def cpu_baseline(g, seconds_target=15.0, chunk=64, seed=12345):
     }
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "intersect_rmat20_traffic.json")
+
+
+def summarize_pmc(fetch_csv, write_csv, calib_bytes, kernel="VerifyKernel"):
+    """HBM bytes per dispatch of `kernel` from two rocprofv3 --pmc runs
+    (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each scaled by what the
+    same counter read for the calibration copy of calib_bytes (the largest
+    __amd_rocclr_copyBuffer dispatch: it reads and writes calib_bytes)."""
+    import csv
+
+    def load(path):
+        rows = list(csv.DictReader(open(path)))
+        copy = max((float(r["Counter_Value"]) for r in rows if "copyBuffer" in r["Kernel_Name"]),
+                   default=0.0)
+        vals = [float(r["Counter_Value"]) for r in rows if kernel in r["Kernel_Name"]]
+        return copy, vals
+
+    fc, fv = load(fetch_csv)
+    wc, wv = load(write_csv)
+    f_factor = (calib_bytes / 1024.0) / fc
+    w_factor = (calib_bytes / 1024.0) / wc
+    fetch = sum(fv) / len(fv) * 1024.0 * f_factor
+    write = sum(wv) / len(wv) * 1024.0 * w_factor
+    return {"kernel": kernel, "dispatches": len(fv), "fetch_kib_raw_per_dispatch": sum(fv) / len(fv),
+            "write_kib_raw_per_dispatch": sum(wv) / len(wv), "fetch_factor": f_factor,
+            "write_factor": w_factor, "fetch_bytes_per_launch": fetch, "write_bytes_per_launch": write,
+            "bytes_per_launch": fetch + write}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,17 +118,30 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first run one device copy of this size (counter calibration)")
-    ap.add_argument("--hub-min-degree", type=int, default=0,
+    ap.add_argument("--hub-min-degree", type=int, default=256,
                     help="adjacency bitmaps for vertices of at least this degree (0: off)")
     ap.add_argument("--hub-max-gb", type=float, default=8.0)
-    ap.add_argument("--workload", choices=("rmat20", "er-batch"), default="rmat20")
+    ap.add_argument("--workload", choices=("rmat20", "er-batch", "serve"), default="rmat20")
+    ap.add_argument("--serve-seconds", type=float, default=60.0)
+    ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-scale", type=int, default=22)
+    ap.add_argument("--serve-edge-factor", type=int, default=32)
+    ap.add_argument("--plain-limit-s", type=float, default=240.0,
+                    help="stop replaying into the plain path after this many seconds")
     ap.add_argument("--er-graphs", type=int, default=20000)
     ap.add_argument("--per-graph-sample", type=int, default=1000,
                     help="members timed with one pass each (er-batch)")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tune", default="",
                     help="lane_path_max,item_grid,batch_bytes,super_chunks schedule setting (rmat20)")
+    ap.add_argument("--summarize-pmc", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "CALIB_MIB"),
+                    help="print HBM traffic per intersect launch from two rocprofv3 --pmc runs")
     args = ap.parse_args()
+    if args.summarize_pmc:
+        f, w, mib = args.summarize_pmc
+        print(json.dumps(summarize_pmc(f, w, int(mib) << 20)))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
@@ -99,6 +150,9 @@ def main():
 
     if args.workload == "er-batch":
         bench_er_batch(args, sme)
+        return
+    if args.workload == "serve":
+        bench_serve(args, sme)
         return
 
     t0 = time.perf_counter()
@@ -192,6 +246,17 @@ def main():
             "launches": dom["launches"],
         },
     }
+    # HBM bytes per launch measured by rocprofv3 counters for this same
+    # configuration (profiles/intersect_rmat20_traffic.json)
+    try:
+        with open(TRAFFIC_FILE) as f:
+            tr = json.load(f)
+        if tr.get("config") == {"hub_min_degree": args.hub_min_degree, "schedule": dev.get_tuning()}:
+            out["roofline"]["traffic"] = tr["bytes_per_launch"]
+            out["roofline"]["traffic_source"] = os.path.relpath(TRAFFIC_FILE, ROOT)
+            out["roofline"]["traffic_gbs"] = tr["bytes_per_launch"] / 1e9 / (avg_ms / 1e3)
+    except (OSError, ValueError, KeyError):
+        pass
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline(g)
@@ -300,6 +365,146 @@ def bench_er_batch(args, sme):
                      "avg_launch_ms": avg_ms, "launches": dom["launches"]},
     }
     dev.close()
+    print(json.dumps(out))
+
+
+SERVE_QUERIES = (0, 1, 2, 3, 4, 7)
+SERVE_WEIGHTS = (0.3, 0.2, 0.2, 0.15, 0.1, 0.05)
+
+
+def request_stream(n, seconds, rate, seed):
+    """Seeded Poisson arrivals; (t, query, v_begin, v_end) with the start
+    range [lo, lo + ceil(f n)), f ~ U[0.01, 1], lo ~ U[0, n - len]."""
+    import math
+
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    out, t = [], 0.0
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t >= seconds:
+            break
+        q = int(rng.choice(SERVE_QUERIES, p=SERVE_WEIGHTS))
+        ln = max(1, math.ceil(rng.uniform(0.01, 1.0) * n))
+        lo = int(rng.integers(0, n - ln + 1))
+        out.append((t, q, lo, lo + ln))
+    return out
+
+
+def replay(srv, stream, limit_s=None):
+    """Submits each request at its arrival time (relative to now) from this
+    thread while a second thread waits for results.  With limit_s, stops
+    submitting after that many seconds.  Returns (results, submitted, wall
+    seconds from the first arrival to the last completion)."""
+    import threading
+
+    t0 = time.perf_counter()
+    ids, results = [], {}
+    done = threading.Event()
+    lock = threading.Condition()
+
+    def waiter():
+        k = 0
+        while True:
+            with lock:
+                while k >= len(ids) and not done.is_set():
+                    lock.wait(0.05)
+                if k >= len(ids) and done.is_set():
+                    return
+                rid = ids[k]
+            results[k] = srv.wait(rid)
+            k += 1
+            if k % 500 == 0:
+                log(f"    {k} requests done")
+
+    th = threading.Thread(target=waiter)
+    th.start()
+    for t, q, a, b in stream:
+        now = time.perf_counter() - t0
+        if limit_s is not None and t > limit_s:
+            break
+        if t > now:
+            time.sleep(t - now)
+        rid = srv.submit(q, a, b)
+        with lock:
+            ids.append(rid)
+            lock.notify()
+    done.set()
+    with lock:
+        lock.notify()
+    th.join()
+    wall = time.perf_counter() - t0 - stream[0][0]
+    return [results[k] for k in range(len(ids))], len(ids), wall
+
+
+def bench_serve(args, sme):
+    import numpy as np
+
+    t0 = time.perf_counter()
+    g = sme.gen_rmat(args.serve_scale, args.serve_edge_factor, 0.57, 0.19, 0.19, SEED)
+    log(f"graph rmat{args.serve_scale} ef{args.serve_edge_factor}: n={g.n} nnz={g.nnz} "
+        f"({time.perf_counter() - t0:.1f}s)")
+    stream = request_stream(g.n, args.serve_seconds, args.serve_rate, seed=SEED)
+    log(f"stream: {len(stream)} requests over {args.serve_seconds:.0f}s")
+    cap = int(args.budget_gb * (1 << 30))
+    hub = dict(hub_min_degree=args.hub_min_degree, hub_max_bytes=int(args.hub_max_gb * (1 << 30)))
+
+    def run(label, **kw):
+        srv = sme.Server(g, mem_cap_bytes=cap, **hub, **kw)
+        log(f"{label}: replaying")
+        res, sent, wall = replay(srv, stream)
+        st = srv.stats()
+        srv.close()
+        lat = np.array([r[1] for r in res])
+        out = {"requests": sent, "wall_s": wall, "throughput_rps": sent / wall,
+               "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+               "mean_ms": float(lat.mean()), "stats": st}
+        log(f"{label}: {out}")
+        return [r[0] for r in res], out
+
+    served_counts, served = run("served", slots=args.serve_slots, merge=True, min_reserve_bytes=4 << 30)
+    # plain path: one request at a time in arrival order, whole budget each
+    srv = sme.Server(g, mem_cap_bytes=cap, slots=1, merge=False, **hub)
+    log("plain: replaying")
+    res, sent, wall = replay(srv, stream, args.plain_limit_s)
+    st = srv.stats()
+    srv.close()
+    lat = np.array([r[1] for r in res])
+    plain = {"requests": sent, "wall_s": wall, "throughput_rps": sent / wall,
+             "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+             "mean_ms": float(lat.mean()), "stats": st,
+             "stream_cut_at_s": args.plain_limit_s if sent < len(stream) else None}
+    log(f"plain: {plain}")
+    plain_counts = [r[0] for r in res]
+    if plain_counts != served_counts[:len(plain_counts)]:
+        bad = sum(a != b for a, b in zip(plain_counts, served_counts))
+        raise AssertionError(f"{bad} requests differ between the served mix and the plain path")
+    out = {
+        "metric": "served requests/sec with p50/p99 latency vs plain (1 GPU)",
+        "value": served["throughput_rps"],
+        "unit": "requests/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 0,
+        "ms_per_step": 1e3 * served["wall_s"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32/u64 integer",
+        "data": "synthetic (seeded R-MAT + seeded Poisson request stream)",
+        "config": {
+            "workload": f"RMAT scale {args.serve_scale} edge factor {args.serve_edge_factor} (n={g.n}, "
+                        f"nnz={g.nnz}); Poisson {args.serve_rate:.0f} req/s for {args.serve_seconds:.0f}s, "
+                        "patterns {Q0,Q1,Q2,Q3,Q4,Q7} weights 0.3/0.2/0.2/0.15/0.1/0.05, "
+                        "start range 1-100% of vertices",
+            "mem_cap_gb": args.budget_gb,
+            "served": served,
+            "plain": plain,
+            "throughput_vs_plain": served["throughput_rps"] / plain["throughput_rps"],
+            "counts_checked": len(plain_counts),
+        },
+    }
     print(json.dumps(out))
 
 

@@ -150,8 +150,8 @@ int sme_count_buckets(sme_ctx* ctx, const sme_graph* g, const sme_plan* p, const
  *                  the resident capacity makes the launch persistent.
  *   super_chunks   64-candidate chunks a wavefront takes at once; when they
  *                  all belong to one path, that path's lists are loaded once
- *                  and the window of another sorted list that the chunks'
- *                  values span is staged in LDS (0 or 1: chunk by chunk).
+ *                  and one short list is staged in LDS (0 or 1: chunk by
+ *                  chunk; a new context starts at 4).
  *   batch_bytes    cap on the device bytes one batch of a level may use
  *                  (0: the level's share of the memory budget).
  * Settings apply to every later call on the context; results never depend
@@ -183,6 +183,52 @@ int sme_autotune(sme_ctx* ctx, const sme_graph* g, const sme_plan* const* plans,
                  const uint32_t* bounds, uint64_t nbuckets, double sample_fraction,
                  sme_tuning* best, double* best_ms, sme_tune_trial* trials, int cap,
                  int* ntrials);
+
+/* ---- server: pattern-count requests over one resident graph ------------
+ * No counterpart in the reference (one pattern per process run,
+ * src/gpu/PatternMatchGPU.cu:17-37).  Requests are (query, start-vertex
+ * range) pairs, counted as sme_count would count them (LIGHT, SPECIAL
+ * compress level, ordering on).  `slots` passes run at once, each on its own
+ * stream, sharing one device pool of mem_cap_bytes.
+ *   merge != 0: continuous batching -- when a slot frees up, every queued
+ *     request of the oldest request's pattern (at most max_merge) becomes
+ *     one pass over the union of their ranges, split per request by
+ *     per-start-vertex buckets.
+ *   admission: a pass starts when its estimated intermediate bytes fit
+ *     beside the passes running (estimate = learned bytes per work unit of
+ *     the pattern x summed squared degrees of the covered start vertices,
+ *     clamped to [min_reserve_bytes, mem_cap_bytes]; a pattern's first pass
+ *     reserves the whole cap).
+ * merge = 0 with one slot is the plain path: one request at a time, in
+ * arrival order, with the whole budget.  Submit and wait may be called from
+ * any host thread. */
+typedef struct sme_server sme_server;
+typedef struct {
+  uint64_t mem_cap_bytes;
+  int32_t slots;
+  int32_t merge;
+  uint64_t max_merge;
+  uint64_t min_reserve_bytes;
+  uint64_t hub_min_degree; /* adjacency bitmaps, as sme_set_hub_bitmaps */
+  uint64_t hub_max_bytes;
+  sme_tuning tuning;       /* schedule of every pass */
+} sme_server_config;
+typedef struct {
+  uint64_t passes;          /* engine passes run */
+  uint64_t requests_done;
+  uint64_t max_group;       /* most requests one pass served */
+  uint64_t peak_reserved;   /* largest sum of admitted reservations (bytes) */
+  uint64_t peak_concurrent; /* most passes running at once */
+  uint64_t failed;
+} sme_server_stats;
+int sme_server_open(int device, const uint64_t* row_ptrs, const uint32_t* cols, uint64_t n,
+                    uint64_t nnz, const sme_server_config* cfg, sme_server** out);
+int sme_server_submit(sme_server* s, int query, uint32_t v_begin, uint32_t v_end, uint64_t* id);
+/* Blocks until request `id` is done; latency_ms = submit to completion. */
+int sme_server_wait(sme_server* s, uint64_t id, uint64_t* count, double* latency_ms);
+int sme_server_get_stats(sme_server* s, sme_server_stats* out);
+/* Finishes every submitted request, then releases the server. */
+void sme_server_close(sme_server* s);
 
 /* ---- kernel profile (HIP events around every launch) ------------------ */
 int sme_profile_enable(sme_ctx* ctx, int on);

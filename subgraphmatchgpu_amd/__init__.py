@@ -65,6 +65,19 @@ class _Trial(ctypes.Structure):
     _fields_ = [("tuning", _Tuning), ("ms", ctypes.c_double)]
 
 
+class _ServerConfig(ctypes.Structure):
+    _fields_ = [("mem_cap_bytes", ctypes.c_uint64), ("slots", ctypes.c_int32), ("merge", ctypes.c_int32),
+                ("max_merge", ctypes.c_uint64), ("min_reserve_bytes", ctypes.c_uint64),
+                ("hub_min_degree", ctypes.c_uint64), ("hub_max_bytes", ctypes.c_uint64),
+                ("tuning", _Tuning)]
+
+
+class _ServerStats(ctypes.Structure):
+    _fields_ = [("passes", ctypes.c_uint64), ("requests_done", ctypes.c_uint64),
+                ("max_group", ctypes.c_uint64), ("peak_reserved", ctypes.c_uint64),
+                ("peak_concurrent", ctypes.c_uint64), ("failed", ctypes.c_uint64)]
+
+
 _lib = None
 
 
@@ -120,6 +133,12 @@ def library() -> ctypes.CDLL:
                               ctypes.c_int),
         "sme_graph_hubs": ([vp, u64p], ctypes.c_int),
         "sme_set_tuning": ([vp, ctypes.POINTER(_Tuning)], ctypes.c_int),
+        "sme_server_open": ([ctypes.c_int, vp, vp, ctypes.c_uint64, ctypes.c_uint64,
+                             ctypes.POINTER(_ServerConfig), ctypes.POINTER(vp)], ctypes.c_int),
+        "sme_server_submit": ([vp, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, u64p], ctypes.c_int),
+        "sme_server_wait": ([vp, ctypes.c_uint64, u64p, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+        "sme_server_get_stats": ([vp, ctypes.POINTER(_ServerStats)], ctypes.c_int),
+        "sme_server_close": ([vp], None),
         "sme_get_tuning": ([vp, ctypes.POINTER(_Tuning)], ctypes.c_int),
         "sme_autotune": ([vp, vp, ctypes.POINTER(vp), ctypes.c_int, vp, ctypes.c_uint64,
                           ctypes.c_double, ctypes.POINTER(_Tuning), ctypes.POINTER(ctypes.c_double),
@@ -429,3 +448,55 @@ def device_count() -> int:
     c = ctypes.c_int()
     _check(library().sme_device_count(ctypes.byref(c)))
     return c.value
+
+
+class Server:
+    """Pattern-count requests over one resident graph (see include/sme.h):
+    continuous batching of same-pattern requests into bucketed passes,
+    admission by estimated intermediate bytes under mem_cap_bytes, `slots`
+    passes in flight.  merge=False, slots=1 is the plain one-at-a-time path."""
+
+    def __init__(self, g: "Graph", mem_cap_bytes: int, slots: int = 4, merge: bool = True,
+                 max_merge: int = 1024, min_reserve_bytes: int = 1 << 30, hub_min_degree: int = 0,
+                 hub_max_bytes: int = 0, tuning: dict | None = None, device: int = 0):
+        lib = library()
+        t = tuning or {}
+        cfg = _ServerConfig(mem_cap_bytes, slots, int(merge), max_merge, min_reserve_bytes,
+                            hub_min_degree, hub_max_bytes,
+                            _Tuning(t.get("lane_path_max", 0), t.get("item_grid", 0),
+                                    t.get("super_chunks", 0), t.get("batch_bytes", 0)))
+        self._rp = np.ascontiguousarray(g.row_ptrs, dtype=np.uint64)
+        self._cols = np.ascontiguousarray(g.cols, dtype=np.uint32)
+        h = ctypes.c_void_p()
+        _check(lib.sme_server_open(device, self._rp.ctypes.data, self._cols.ctypes.data, g.n, g.nnz,
+                                   ctypes.byref(cfg), ctypes.byref(h)))
+        self._h = h
+        self.n = g.n
+
+    def submit(self, query: int, v_begin: int = 0, v_end: int | None = None) -> int:
+        rid = ctypes.c_uint64()
+        _check(library().sme_server_submit(self._h, query, v_begin, self.n if v_end is None else v_end,
+                                           ctypes.byref(rid)))
+        return rid.value
+
+    def wait(self, rid: int) -> tuple:
+        """(count, latency_ms) of request rid, blocking until it is done."""
+        c, ms = ctypes.c_uint64(), ctypes.c_double()
+        _check(library().sme_server_wait(self._h, rid, ctypes.byref(c), ctypes.byref(ms)))
+        return int(c.value), float(ms.value)
+
+    def stats(self) -> dict:
+        st = _ServerStats()
+        _check(library().sme_server_get_stats(self._h, ctypes.byref(st)))
+        return {k: int(getattr(st, k)) for k, _ in _ServerStats._fields_}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            library().sme_server_close(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -13,6 +13,7 @@
 #include "engine.h"
 #include "graph.h"
 #include "query.h"
+#include "serve.h"
 
 struct sme_ctx {
   sme::Context impl;
@@ -23,6 +24,9 @@ struct sme_graph {
 struct sme_plan {
   sme::EnginePlan plan;
   std::string text;
+};
+struct sme_server {
+  sme::Server impl;
 };
 
 namespace {
@@ -331,6 +335,62 @@ int sme_autotune(sme_ctx* ctx, const sme_graph* g, const sme_plan* const* plans,
   if (ntrials) *ntrials = n;
   return SME_OK;
 }
+
+int sme_server_open(int device, const uint64_t* row_ptrs, const uint32_t* cols, uint64_t n,
+                    uint64_t nnz, const sme_server_config* cfg, sme_server** out) {
+  if (!cfg || !out) return Report(nullptr, SME_ERR_INVALID, "null argument");
+  *out = nullptr;
+  sme_server* s = new (std::nothrow) sme_server;
+  if (!s) return Report(nullptr, SME_ERR_OOM, "host allocation failed");
+  sme::ServerConfig c;
+  c.mem_cap = cfg->mem_cap_bytes;
+  c.slots = cfg->slots;
+  c.merge = cfg->merge != 0;
+  c.max_merge = cfg->max_merge;
+  c.min_reserve = cfg->min_reserve_bytes;
+  c.hub_min_degree = cfg->hub_min_degree;
+  c.hub_max_bytes = cfg->hub_max_bytes;
+  c.tuning.lane_path_max = cfg->tuning.lane_path_max;
+  c.tuning.item_grid = cfg->tuning.item_grid;
+  c.tuning.super_chunks = cfg->tuning.super_chunks;
+  c.tuning.batch_bytes = cfg->tuning.batch_bytes;
+  std::string err;
+  const sme::Status st = s->impl.Open(device, row_ptrs, cols, n, nnz, c, &err);
+  if (st != sme::kOk) {
+    delete s;
+    return Report(nullptr, st, err);
+  }
+  *out = s;
+  return SME_OK;
+}
+
+int sme_server_submit(sme_server* s, int query, uint32_t v_begin, uint32_t v_end, uint64_t* id) {
+  if (!s || !id) return Report(nullptr, SME_ERR_INVALID, "null argument");
+  std::string err;
+  const sme::Status st = s->impl.Submit(query, v_begin, v_end, id, &err);
+  return st == sme::kOk ? SME_OK : Report(nullptr, st, err);
+}
+
+int sme_server_wait(sme_server* s, uint64_t id, uint64_t* count, double* latency_ms) {
+  if (!s) return Report(nullptr, SME_ERR_INVALID, "null argument");
+  std::string err;
+  const sme::Status st = s->impl.Wait(id, count, latency_ms, &err);
+  return st == sme::kOk ? SME_OK : Report(nullptr, st, err);
+}
+
+int sme_server_get_stats(sme_server* s, sme_server_stats* out) {
+  if (!s || !out) return SME_ERR_INVALID;
+  const sme::ServerStats st = s->impl.Stats();
+  out->passes = st.passes;
+  out->requests_done = st.requests_done;
+  out->max_group = st.max_group;
+  out->peak_reserved = st.peak_reserved;
+  out->peak_concurrent = st.peak_concurrent;
+  out->failed = st.failed;
+  return SME_OK;
+}
+
+void sme_server_close(sme_server* s) { delete s; }
 
 int sme_set_hub_bitmaps(sme_ctx* ctx, uint64_t min_degree, uint64_t max_bytes) {
   if (!ctx) return SME_ERR_INVALID;

@@ -15,6 +15,7 @@ using uintE = uint64_t;  // edge offset (Meta.h:9)
 
 constexpr int kMaxQueryVertices = 6;  // Meta.h:30
 constexpr int kMaxOps = 16;
+constexpr int kNumQueries = 14;  // QueryType Q0..Q13, Meta.h:104-124
 
 // Status codes returned through the C ABI (include/sme.h).
 enum Status : int {

@@ -27,6 +27,55 @@ int main(int argc, char** argv) {
   uint64_t ve = std::strtoull(argv[7], nullptr, 10);
   if (ve > n) ve = n;
   const uint64_t budget = std::strtoull(argv[8], nullptr, 10);
+  // SERVE="q:vb:ve,q:vb:ve,..." runs the requests through a one-slot server
+  // (SERVE_MERGE=0: plain path) and prints "serve=<c1>,<c2>,..."
+  if (const char* env = std::getenv("SERVE")) {
+    std::vector<uint32_t> q, vbs, ves;
+    for (const char* p = env; *p;) {
+      char* e = nullptr;
+      q.push_back(static_cast<uint32_t>(std::strtoul(p, &e, 10)));
+      vbs.push_back(static_cast<uint32_t>(std::strtoul(e + 1, &e, 10)));
+      ves.push_back(static_cast<uint32_t>(std::strtoul(e + 1, &e, 10)));
+      p = *e == ',' ? e + 1 : e;
+    }
+    sme_server_config cfg{};
+    cfg.mem_cap_bytes = budget;
+    cfg.slots = 1;
+    const char* m = std::getenv("SERVE_MERGE");
+    cfg.merge = m ? std::atoi(m) : 1;
+    cfg.max_merge = 64;
+    cfg.min_reserve_bytes = budget / 4;
+    sme_server* srv = nullptr;
+    if (sme_server_open(0, rp, cols, n, nnz, &cfg, &srv) != SME_OK) {
+      std::fprintf(stderr, "server: %s\n", sme_last_error(nullptr));
+      return 1;
+    }
+    std::vector<uint64_t> ids(q.size());
+    for (size_t i = 0; i < q.size(); ++i) {
+      if (sme_server_submit(srv, static_cast<int>(q[i]), vbs[i], ves[i], &ids[i]) != SME_OK) {
+        std::fprintf(stderr, "submit: %s\n", sme_last_error(nullptr));
+        return 1;
+      }
+    }
+    std::printf("serve=");
+    for (size_t i = 0; i < q.size(); ++i) {
+      uint64_t c = 0;
+      double ms = 0;
+      if (sme_server_wait(srv, ids[i], &c, &ms) != SME_OK) {
+        std::fprintf(stderr, "wait: %s\n", sme_last_error(nullptr));
+        return 1;
+      }
+      std::printf(i ? ",%llu" : "%llu", static_cast<unsigned long long>(c));
+    }
+    sme_server_stats st{};
+    sme_server_get_stats(srv, &st);
+    std::printf(" passes=%llu max_group=%llu\n", static_cast<unsigned long long>(st.passes),
+                static_cast<unsigned long long>(st.max_group));
+    sme_server_close(srv);
+    sme_free_host(rp);
+    sme_free_host(cols);
+    return 0;
+  }
   sme_ctx* ctx = nullptr;
   if (sme_open(0, budget, &ctx) != SME_OK) {
     std::fprintf(stderr, "open: %s\n", sme_last_error(nullptr));

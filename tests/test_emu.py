@@ -186,3 +186,30 @@ def test_emu_super_chunk_fast_path(emu, tmp_path):
             parts = r.stdout.split()
             assert int(parts[0].split("=")[1]) == want[0], (i, q, tune)
             assert [int(x) for x in parts[1].split("=")[1].split(",")] == want[1], (i, q, tune)
+
+
+@pytest.mark.parametrize("merge", ["1", "0"])
+def test_emu_server_requests(emu, graphs, merge):
+    """Requests through the server (merged bucketed passes, or the plain
+    one-at-a-time path) get the oracle's count for their range."""
+    import numpy as np
+
+    g, p = graphs["rmat7"]
+    rng = np.random.default_rng(7)
+    reqs = []
+    for _ in range(24):
+        q = int(rng.choice([0, 1, 2, 3, 4, 7]))
+        a = int(rng.integers(0, g.n))
+        reqs.append((q, a, int(rng.integers(a, g.n + 1))))
+    reqs += [(0, 5, 5), (3, 0, g.n), (3, 0, g.n)]
+    want = [oracle.light_count(g.row_ptrs, g.cols, q, 3, v_begin=a, v_end=b)[0] for q, a, b in reqs]
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0", SERVE_MERGE=merge,
+               SERVE=",".join(f"{q}:{a}:{b}" for q, a, b in reqs))
+    r = subprocess.run([emu, p, "6", "0", "3", "1", "0", str(g.n), str(1 << 30)],
+                       capture_output=True, text=True, env=env, timeout=600)
+    assert r.returncode == 0, r.stderr[-2000:]
+    got = [int(x) for x in r.stdout.split()[0].split("=")[1].split(",")]
+    assert got == want
+    passes = int(r.stdout.split()[1].split("=")[1])
+    nonempty = sum(b > a for _, a, b in reqs)
+    assert passes < nonempty if merge == "1" else passes == nonempty

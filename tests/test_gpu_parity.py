@@ -284,3 +284,34 @@ def test_autotune_installs_a_measured_setting(fixture_graphs):
     best, ms, trials = d.autotune(dgb, [sme.Plan(sme.Q0), sme.Plan(sme.Q1)], bounds=offs, sample_fraction=0.25)
     assert d.get_tuning() == best
     d.close()
+
+
+@pytest.mark.parametrize("slots,merge,hub", [(4, True, 0), (4, True, 64), (1, False, 0), (2, True, 0)])
+def test_server_results_alone_and_in_the_mix(fixture_graphs, slots, merge, hub):
+    """Each request's count from the server (merged bucketed passes running
+    concurrently on several streams of one shared pool) equals its count
+    alone through sme_count."""
+    g = fixture_graphs["rmat12"]
+    rng = np.random.default_rng(100 + slots)
+    reqs = []
+    for _ in range(120):
+        q = int(rng.choice([0, 1, 2, 3, 4, 7], p=[0.3, 0.2, 0.2, 0.15, 0.1, 0.05]))
+        ln = max(1, int(rng.uniform(0.01, 1.0) * g.n))
+        a = int(rng.integers(0, g.n - ln + 1))
+        reqs.append((q, a, a + ln))
+    d = sme.Device(0, budget_bytes=2 << 30)
+    dg = d.upload(g)
+    alone = [d.count(dg, sme.Plan(q), a, b).count for q, a, b in reqs]
+    d.close()
+    say(f"case server slots={slots} merge={merge} hub={hub}")
+    srv = sme.Server(g, mem_cap_bytes=2 << 30, slots=slots, merge=merge, min_reserve_bytes=64 << 20,
+                     hub_min_degree=hub, hub_max_bytes=1 << 30 if hub else 0)
+    ids = [srv.submit(q, a, b) for q, a, b in reqs]
+    got = [srv.wait(i)[0] for i in ids]
+    st = srv.stats()
+    srv.close()
+    assert got == alone
+    assert st["requests_done"] == len(reqs) and st["failed"] == 0
+    if merge:
+        assert st["passes"] < len(reqs)
+    assert st["peak_reserved"] <= 2 << 30
