@@ -315,3 +315,31 @@ def test_server_results_alone_and_in_the_mix(fixture_graphs, slots, merge, hub):
     if merge:
         assert st["passes"] < len(reqs)
     assert st["peak_reserved"] <= 2 << 30
+
+
+def test_config3_rmat22_slices():
+    """RMAT scale 22, edge factor 32 (the memory-capped configuration), full
+    CSR resident, 5-clique and house on start-vertex slices: counts and
+    per-level outputs equal the LIGHT oracle where it finishes in seconds;
+    on a heavier slice a 32 GB and a 64 MB budget give identical counts and
+    levels (the smaller in more batches) and GPSM_REUSE agrees on the
+    count."""
+    g = sme.gen_rmat(22, 32, seed=1)
+    assert g.n == 1 << 22
+    big = sme.Device(0, budget_bytes=32 << 30)
+    small = sme.Device(0, budget_bytes=64 << 20)
+    gb, gs = big.upload(g), small.upload(g)
+    for q, (a, b) in ((sme.Q7, (3 << 20, (3 << 20) + 64)), (sme.Q4, (2500000, 2500256))):
+        say(f"case config3 oracle q={q} range=[{a},{b})")
+        r = big.count(gb, sme.Plan(q), a, b)
+        assert (r.count, r.levels) == oracle.light_count(g.row_ptrs, g.cols, q, 3, v_begin=a, v_end=b)
+    for q, (a, b) in ((sme.Q7, (3 << 20, (3 << 20) + 256)), (sme.Q4, (2500000, 2501024))):
+        say(f"case config3 budgets q={q} range=[{a},{b})")
+        r1 = big.count(gb, sme.Plan(q), a, b)
+        r2 = small.count(gs, sme.Plan(q), a, b)
+        assert (r1.count, r1.levels) == (r2.count, r2.levels)
+        assert r2.batches > r1.batches
+        if q == sme.Q7:
+            assert big.count(gb, sme.Plan(q, sme.GPU_GPSM_REUSE), a, b).count == r1.count
+    big.close()
+    small.close()
