@@ -132,14 +132,14 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-scale", type=int, default=16)
     ap.add_argument("--serve-edge-factor", type=int, default=16)
-    ap.add_argument("--plain-limit-s", type=float, default=10.0,
+    ap.add_argument("--plain-limit-s", type=float, default=2.0,
                     help="replay only this many seconds of the stream into the plain path "
                          "(it cannot keep up; its backlog drains after the cut)")
     ap.add_argument("--er-graphs", type=int, default=20000)
     ap.add_argument("--per-graph-sample", type=int, default=1000,
                     help="members timed with one pass each (er-batch)")
     ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--tune", default="",
+    ap.add_argument("--tune", default="0,0,0,4",
                     help="lane_path_max,item_grid,batch_bytes,super_chunks schedule setting (rmat20)")
     ap.add_argument("--summarize-pmc", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "CALIB_MIB"),
                     help="print HBM traffic per intersect launch from two rocprofv3 --pmc runs")

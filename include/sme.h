@@ -137,6 +137,21 @@ int sme_count_buckets(sme_ctx* ctx, const sme_graph* g, const sme_plan* p, const
                       uint64_t nbuckets, uint64_t* counts, uint64_t* out_total,
                       uint64_t* level_counts, sme_stats* stats);
 
+/* Match list of a full-materialization plan (SME_ALGO_GPSM or
+ * SME_ALGO_GPSM_REUSE) over start vertices [v_begin, v_end): each match is a
+ * row of sme_plan_vertices(p) u32, the data vertex of query vertex 0, 1, ...,
+ * and rows come in lexicographic order (the order GPSM's levels produce
+ * them).  At most max_matches rows are written to out; *total receives the
+ * full count and *written the rows written.  A LIGHT plan is
+ * SME_ERR_UNSUPPORTED (it counts without materializing every vertex).  No
+ * counterpart in the reference: its -l flag is parsed
+ * (src/gpu/PatternMatchGPU.cu:29) but never consumed. */
+int sme_list_matches(sme_ctx* ctx, const sme_graph* g, const sme_plan* p, uint32_t v_begin,
+                     uint32_t v_end, uint32_t* out, uint64_t max_matches, uint64_t* total,
+                     uint64_t* written);
+/* Query vertices of the plan's pattern (row width of sme_list_matches). */
+int sme_plan_vertices(const sme_plan* p);
+
 /* ---- schedule tuning ---------------------------------------------------
  * No counterpart in the reference, whose launch shapes are compile-time
  * constants (gpu/utils/ops/GPUIntersection.cuh, LIGHTCommon.cuh).
@@ -151,7 +166,7 @@ int sme_count_buckets(sme_ctx* ctx, const sme_graph* g, const sme_plan* p, const
  *   super_chunks   64-candidate chunks a wavefront takes at once; when they
  *                  all belong to one path, that path's lists are loaded once
  *                  and one short list is staged in LDS (0 or 1: chunk by
- *                  chunk; a new context starts at 4).
+ *                  chunk, the default).
  *   batch_bytes    cap on the device bytes one batch of a level may use
  *                  (0: the level's share of the memory budget).
  * Settings apply to every later call on the context; results never depend

@@ -502,6 +502,9 @@ typedef struct {
   uint32_t* buf[OR_MAXQ];
   uint32_t* tmp;
   uint64_t level[OR_MAXQ];
+  /* match-list output (sequential walk only) */
+  uint32_t* out;
+  uint64_t cap, rows;
 } gpsm_ctx;
 
 static void gpsm_dfs(gpsm_ctx* c, int lvl, uint32_t* path) {
@@ -516,6 +519,10 @@ static void gpsm_dfs(gpsm_ctx* c, int lvl, uint32_t* path) {
     if (!ok) continue;
     path[lvl] = x;
     c->level[lvl]++;
+    if (lvl == c->nv - 1 && c->out && c->rows < c->cap) {
+      memcpy(c->out + c->rows * (uint64_t)c->nv, path, sizeof(uint32_t) * (size_t)c->nv);
+      c->rows++;
+    }
     gpsm_dfs(c, lvl + 1, path);
   }
 }
@@ -562,6 +569,52 @@ int or_gpsm_count(uint64_t n, const uint64_t* row_ptrs, const uint32_t* cols, in
   }
   *out_count = lv_tot[qq.nv - 1];
   if (level_out) for (int u = 0; u < qq.nv; ++u) level_out[u] = lv_tot[u];
+  return 0;
+}
+
+/* The GPSM matches themselves, in the order a sequential depth-first walk
+ * over sorted candidate sets visits them: lexicographic in (M[0], M[1],
+ * ...).  Pinned without re-typing: the row count equals or_gpsm_count, and
+ * the tests check every row against the pattern's edges and conditions
+ * directly and check that rows are distinct and sorted, which together fix
+ * the list. */
+int or_gpsm_list(uint64_t n, const uint64_t* row_ptrs, const uint32_t* cols, int q,
+                 int enable_ordering, uint32_t v_begin, uint32_t v_end, uint32_t* out,
+                 uint64_t cap, uint64_t* rows, uint64_t* total) {
+  or_query qq;
+  if (or_query_init(q, enable_ordering, &qq)) return -1;
+  if (v_end > n) v_end = (uint32_t)n;
+  for (int u = 1; u < qq.nv; ++u) {
+    int has = 0;
+    for (int j = 0; j < qq.con[u].cnt; ++j) has |= qq.con[u].v[j] < u;
+    if (!has) return -2;
+  }
+  graph_t g = {n, row_ptrs, cols};
+  uint64_t md = max_degree(&g) + 1;
+  gpsm_ctx c;
+  memset(&c, 0, sizeof(c));
+  c.g = &g;
+  c.nv = qq.nv;
+  c.out = out;
+  c.cap = cap;
+  for (int u = 0; u < qq.nv; ++u) {
+    for (int j = 0; j < qq.con[u].cnt; ++j)
+      if (qq.con[u].v[j] < u) c.lv[u][c.nlv[u]++] = qq.con[u].v[j];
+    for (int v = 0; v < u; ++v) cs_push(&c.cond[u], cond_type(qq.order, u, v), v);
+    c.buf[u] = (uint32_t*)malloc(md * sizeof(uint32_t));
+  }
+  c.tmp = (uint32_t*)malloc(md * sizeof(uint32_t));
+  uint32_t path[OR_MAXQ];
+  for (uint64_t v = v_begin; v < v_end; ++v) {
+    path[0] = (uint32_t)v;
+    c.level[0]++;
+    if (c.nv == 1 && c.out && c.rows < c.cap) c.out[c.rows++] = (uint32_t)v;
+    gpsm_dfs(&c, 1, path);
+  }
+  for (int u = 0; u < qq.nv; ++u) free(c.buf[u]);
+  free(c.tmp);
+  *rows = c.rows;
+  *total = c.level[qq.nv - 1];
   return 0;
 }
 

@@ -99,6 +99,10 @@ int or_light_count(uint64_t n, const uint64_t* row_ptrs, const uint32_t* cols,
 int or_gpsm_count(uint64_t n, const uint64_t* row_ptrs, const uint32_t* cols,
                   int q, int enable_ordering, uint32_t v_begin, uint32_t v_end,
                   int nthreads, uint64_t* out_count, uint64_t* level_out);
+/* GPSM match list, lexicographic; up to cap rows of nv u32 into out. */
+int or_gpsm_list(uint64_t n, const uint64_t* row_ptrs, const uint32_t* cols, int q,
+                 int enable_ordering, uint32_t v_begin, uint32_t v_end, uint32_t* out,
+                 uint64_t cap, uint64_t* rows, uint64_t* total);
 
 #ifdef __cplusplus
 }

@@ -138,3 +138,23 @@ def ref_count(bin_path: str, q: int, algo: int = 0, threads: int = 1, buffer_gb:
             a, b = line.split(",")
             return int(a.split("=")[1]), float(b.split("=")[1].rstrip("ms"))
     raise RuntimeError(f"reference run failed: rc={r.returncode} {r.stdout[-400:]} {r.stderr[-400:]}")
+
+
+def gpsm_list(rp, cols, q, v_begin=0, v_end=None, cap=1 << 20, enable_ordering=True):
+    """GPSM matches as an (rows, nv) uint32 array (lexicographic) and the
+    full count."""
+    rp, cols = _arrays(rp, cols)
+    n = rp.size - 1
+    nv = (3, 4, 4, 4, 5, 6, 5, 5, 5, 5, 5, 6, 5, 6)[q]
+    out = np.zeros((max(cap, 1), nv), dtype=np.uint32)
+    rows, total = ctypes.c_uint64(), ctypes.c_uint64()
+    L = lib()
+    L.or_gpsm_list.argtypes = [ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                               ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint64,
+                               ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]
+    rc = L.or_gpsm_list(n, rp.ctypes.data, cols.ctypes.data, q, int(enable_ordering), v_begin,
+                        n if v_end is None else v_end, out.ctypes.data, cap, ctypes.byref(rows),
+                        ctypes.byref(total))
+    if rc:
+        raise ValueError(rc)
+    return out[:rows.value], int(total.value)

@@ -133,6 +133,9 @@ def library() -> ctypes.CDLL:
                               ctypes.c_int),
         "sme_graph_hubs": ([vp, u64p], ctypes.c_int),
         "sme_set_tuning": ([vp, ctypes.POINTER(_Tuning)], ctypes.c_int),
+        "sme_list_matches": ([vp, vp, vp, ctypes.c_uint32, ctypes.c_uint32, vp, ctypes.c_uint64, u64p, u64p],
+                             ctypes.c_int),
+        "sme_plan_vertices": ([vp], ctypes.c_int),
         "sme_server_open": ([ctypes.c_int, vp, vp, ctypes.c_uint64, ctypes.c_uint64,
                              ctypes.POINTER(_ServerConfig), ctypes.POINTER(vp)], ctypes.c_int),
         "sme_server_submit": ([vp, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, u64p], ctypes.c_int),
@@ -363,6 +366,19 @@ class Device:
         """Adjacency bitmaps for vertices of degree >= min_degree in graphs
         uploaded afterwards (0 disables)."""
         _check(library().sme_set_hub_bitmaps(self._h, min_degree, max_bytes), self._h)
+
+    def list_matches(self, dg: "DeviceGraph", plan: Plan, v_begin: int = 0, v_end: int | None = None,
+                     max_matches: int = 1 << 20):
+        """Matches of a GPSM plan as an (rows, nv) uint32 array in
+        lexicographic order (at most max_matches rows) and the full count."""
+        lib = library()
+        nv = lib.sme_plan_vertices(plan._h)
+        out = np.zeros((max(max_matches, 1), nv), dtype=np.uint32)
+        total, rows = ctypes.c_uint64(), ctypes.c_uint64()
+        ve = dg.n if v_end is None else v_end
+        _check(lib.sme_list_matches(self._h, dg._h, plan._h, v_begin, ve, out.ctypes.data, max_matches,
+                                    ctypes.byref(total), ctypes.byref(rows)), self._h)
+        return out[:rows.value], int(total.value)
 
     def set_tuning(self, lane_path_max: int = 0, item_grid: int = 0, batch_bytes: int = 0,
                    super_chunks: int = 0):

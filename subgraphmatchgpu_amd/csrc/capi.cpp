@@ -206,6 +206,18 @@ int sme_count_buckets(sme_ctx* ctx, const sme_graph* g, const sme_plan* p, const
   return SME_OK;
 }
 
+int sme_list_matches(sme_ctx* ctx, const sme_graph* g, const sme_plan* p, uint32_t v_begin,
+                     uint32_t v_end, uint32_t* out, uint64_t max_matches, uint64_t* total,
+                     uint64_t* written) {
+  if (!ctx || !g || !p || !total || !written) return Report(ctx, SME_ERR_INVALID, "null argument");
+  std::string err;
+  const sme::Status s = sme::ListMatches(&ctx->impl, &g->impl, &p->plan, v_begin, v_end, out,
+                                         max_matches, total, written, &err);
+  return s == sme::kOk ? SME_OK : Report(ctx, s, err);
+}
+
+int sme_plan_vertices(const sme_plan* p) { return p ? p->plan.nv : -1; }
+
 int sme_set_tuning(sme_ctx* ctx, const sme_tuning* t) {
   if (!ctx || !t) return Report(ctx, SME_ERR_INVALID, "null argument");
   ctx->impl.tuning.lane_path_max = t->lane_path_max;

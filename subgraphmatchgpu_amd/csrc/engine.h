@@ -89,14 +89,14 @@ class Context {
   //                  paths use the item schedule)
   //   item_grid      workgroup cap of the item kernels (0: default)
   //   super_chunks   chunks per wave in the verify kernels (single-path
-  //                  runs take the staged fast path; 0 or 1: off; 4 by
-  //                  default, the fastest on R-MAT scale 20)
+  //                  runs take the staged fast path; 0 or 1: off, the
+  //                  default; 4 is the fastest on R-MAT scale 20)
   //   batch_bytes    cap on the device bytes of one batch (0: the level's
   //                  share of the pool)
   struct Tuning {
     uint64_t lane_path_max = 0;
     uint32_t item_grid = 0;
-    uint32_t super_chunks = 4;
+    uint32_t super_chunks = 0;
     uint64_t batch_bytes = 0;
   } tuning;
 
@@ -154,5 +154,13 @@ Status RunPlan(Context* ctx, const DevGraph* g, const EnginePlan* plan, uint32_t
 Status RunPlanBuckets(Context* ctx, const DevGraph* g, const EnginePlan* plan,
                       const uint32_t* bounds, uint64_t nbuckets, uint64_t* counts, uint64_t* count,
                       uint64_t* level_counts, RunStats* stats, std::string* err);
+
+// Match list of a full-materialization plan (GPSM / GPSM_REUSE) over start
+// vertices [v_begin, v_end): up to max_rows tuples of plan->nv u32 written
+// to host memory `out` in lexicographic order; *total = the full count,
+// *rows = tuples written.
+Status ListMatches(Context* ctx, const DevGraph* g, const EnginePlan* plan, uint32_t v_begin,
+                   uint32_t v_end, uint32_t* out, uint64_t max_rows, uint64_t* total, uint64_t* rows,
+                   std::string* err);
 
 }  // namespace sme

@@ -279,6 +279,17 @@ class Runner {
   Runner(Context* ctx, const DevGraph* g, const EnginePlan* plan)
       : ctx_(ctx), g_(g), plan_(plan), st_(ctx->stream()), pool_(ctx) {}
 
+  // Match-list output (full-materialization plans): the tuples of the last
+  // level are appended to host_out (row-major, plan vertex order) up to cap
+  // rows, in the order the batches finish, which is lexicographic.
+  void SetMatchOutput(uint32_t* host_out, uint64_t cap) {
+    list_ = true;
+    list_out_ = host_out;
+    list_cap_ = cap;
+    list_rows_ = 0;
+  }
+  uint64_t ListedRows() const { return list_rows_; }
+
   // Per-bucket counts (device arrays owned by the caller).
   void SetBuckets(const uint32_t* dev_bounds, uint64_t nbuckets, unsigned long long* dev_counts) {
     bk_.bounds = dev_bounds;
@@ -335,6 +346,8 @@ class Runner {
       for (int i = 0; i < op.nc; ++i) ref |= 1u << op.cond[i].v;
       if (op.kind == kCompute || op.kind == kComputePathCount || op.kind == kComputeCount)
         for (int i = 0; i < op.nb; ++i) ref |= 1u << op.bconn[i];
+      // match lists read every column at the last level
+      if (list_) live_after_[L] |= (1u << plan_->nv) - 1;
       // per-bucket counting reads the first matched vertex at every level
       if (bk_.bounds) {
         ref |= 1u << bk_.start_v;
@@ -962,6 +975,19 @@ class Runner {
         f2.has_reuse = true;
       }
       const bool gpsm = plan_->algo == kAlgoGpsm || plan_->algo == kAlgoGpsmReuse;
+      if (gpsm && list_ && L + 1 == plan_->nops && C && list_rows_ < list_cap_) {
+        const uint64_t rows = std::min<uint64_t>(C, list_cap_ - list_rows_);
+        Buf<uint32_t> tuples(pool_, rows * plan_->nv);
+        {
+          Timed t(ctx_, st_, kSlotExpand, 4.0 * rows * plan_->nv * 2, rows);
+          LaunchGatherTuples(f2.cs, plan_->nv, rows, tuples.get(), st_);
+        }
+        SME_HIP(hipMemcpyAsync(list_out_ + list_rows_ * plan_->nv, tuples.get(),
+                               rows * plan_->nv * sizeof(uint32_t), hipMemcpyDeviceToHost, st_));
+        SME_HIP(hipStreamSynchronize(st_));
+        stats_.d2h_syncs++;
+        list_rows_ += rows;
+      }
       if (gpsm && bk_.bounds && L + 1 == plan_->nops && C) {
         // GPSM counts the last level's partial instances: one per path
         CountPathArgs ca{};
@@ -1126,6 +1152,9 @@ class Runner {
   hipStream_t st_;
   Context* pool_;  // allocations of this run go through the context
   unsigned long long* acc_ = nullptr;
+  bool list_ = false;
+  uint32_t* list_out_ = nullptr;
+  uint64_t list_cap_ = 0, list_rows_ = 0;
   int lane_lists_[kMaxOps];  // lists per item of lane-counted levels
   uint64_t host_level_[kMaxOps];
   uint64_t levels_[kMaxOps];
@@ -1170,6 +1199,44 @@ Status RunPlan(Context* ctx, const DevGraph* g, const EnginePlan* plan, uint32_t
     if (stats) *stats = st;
   } catch (const Error& e) {
     // drain the stream so pool memory handed back is no longer in use
+    (void)hipStreamSynchronize(ctx->stream());
+    for (auto& m : ctx->marks) {
+      ctx->spare_events.push_back(m.start);
+      ctx->spare_events.push_back(m.stop);
+    }
+    ctx->marks.clear();
+    *err = e.msg;
+    return e.status;
+  }
+  return kOk;
+}
+
+Status ListMatches(Context* ctx, const DevGraph* g, const EnginePlan* plan, uint32_t v_begin,
+                   uint32_t v_end, uint32_t* out, uint64_t max_rows, uint64_t* total, uint64_t* rows,
+                   std::string* err) {
+  if (!ctx || !g || !plan || !total || !rows || (max_rows && !out)) {
+    *err = "null argument";
+    return kErrInvalid;
+  }
+  if (plan->algo != kAlgoGpsm && plan->algo != kAlgoGpsmReuse) {
+    *err = "match lists need a full-materialization plan (GPU_GPSM or GPU_GPSM_REUSE)";
+    return kErrUnsupported;
+  }
+  if (g->device != ctx->device()) {
+    *err = "graph lives on another device";
+    return kErrInvalid;
+  }
+  if (v_end > g->n) v_end = static_cast<uint32_t>(g->n);
+  try {
+    SME_HIP(hipSetDevice(ctx->device()));
+    ctx->held_peak = ctx->held;
+    Runner r(ctx, g, plan);
+    r.SetMatchOutput(out, max_rows);
+    RunStats st;
+    r.Run(v_begin, v_end, total, nullptr, &st);
+    if (ctx->profiling) ctx->ResolveMarks();
+    *rows = r.ListedRows();
+  } catch (const Error& e) {
     (void)hipStreamSynchronize(ctx->stream());
     for (auto& m : ctx->marks) {
       ctx->spare_events.push_back(m.start);

@@ -218,6 +218,8 @@ constexpr uint32_t kNoHub = 0xffffffffu;
 // Sets bit x of row h for every x in N(hubs[h]); rows must be zero.
 void LaunchBuildHubBits(const uint64_t* rp, const uint32_t* cols, const uint32_t* hubs,
                         uint64_t nhub, uint64_t words, uint32_t* bits, hipStream_t stream);
+// out[p * nv + w] = column w of path p (nv columns, all materialized).
+void LaunchGatherTuples(const ColSet& cs, int nv, uint64_t n, uint32_t* out, hipStream_t stream);
 void LaunchFill32(uint32_t* out, uint64_t n, uint32_t value, hipStream_t stream);
 void LaunchIota32(uint32_t* out, uint64_t n, uint32_t base, hipStream_t stream);
 void LaunchFill64(uint64_t* out, uint64_t n, uint64_t value, hipStream_t stream);

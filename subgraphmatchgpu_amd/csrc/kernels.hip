@@ -742,6 +742,16 @@ __global__ __launch_bounds__(kThreads) void ExpandKernel(ExpandArgs a) {
 }
 
 // One work-group per hub: OR each neighbour's bit into the hub's row.
+// Match-list output: row p of `out` is (M[0], ..., M[nv-1]) of path p.
+__global__ __launch_bounds__(kThreads) void GatherTuplesKernel(ColSet cs, int nv, uint64_t n,
+                                                               uint32_t* out) {
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n * nv;
+       i += GridStride()) {
+    const uint64_t p = i / nv;
+    out[i] = Gather(cs, static_cast<int>(i - p * nv), p);
+  }
+}
+
 __global__ __launch_bounds__(kThreads) void BuildHubBitsKernel(const uint64_t* rp, const uint32_t* cols,
                                                                const uint32_t* hubs, uint64_t nhub,
                                                                uint64_t words, uint32_t* bits) {
@@ -963,6 +973,12 @@ void LaunchBuildHubBits(const uint64_t* rp, const uint32_t* cols, const uint32_t
   if (nhub == 0) return;
   hipLaunchKernelGGL(BuildHubBitsKernel, dim3(GridFor(nhub, 1)), dim3(kThreads), 0, stream, rp, cols, hubs,
                      nhub, words, bits);
+}
+
+void LaunchGatherTuples(const ColSet& cs, int nv, uint64_t n, uint32_t* out, hipStream_t stream) {
+  if (n == 0 || nv <= 0) return;
+  hipLaunchKernelGGL(GatherTuplesKernel, dim3(GridFor(n * nv, kThreads)), dim3(kThreads), 0, stream, cs, nv,
+                     n, out);
 }
 
 void LaunchFill32(uint32_t* out, uint64_t n, uint32_t value, hipStream_t stream) {

@@ -13,11 +13,11 @@
 // CudaContext.cuh:152-163).
 //
 // Shared mode (several contexts, each with its own stream, carving one
-// arena, as the slots of a server do): a block freed by one stream is
-// handed to another stream only after an event recorded at the free has
-// completed, so a kernel of the new owner never overwrites data a kernel of
-// the old owner may still read.  The freeing stream itself may reuse the
-// block at once.  All public members lock the pool.
+// arena, as the slots of a server do): a freed block returns to the free
+// list only after an event recorded on the freeing stream at the free has
+// completed, so a kernel of the next owner (on any stream) never overwrites
+// data a kernel of the old owner may still read.  All public members lock
+// the pool.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -63,8 +63,8 @@ class DevicePool {
   void* AllocLocked(size_t bytes);
   void FreeLocked(void* p);
   size_t LargestFreeLocked() const;
-  // Returns deferred blocks whose free event completed (or that `stream`
-  // freed itself); with `wait`, waits for all of them.
+  // Returns deferred blocks whose free event completed; with `wait`, waits
+  // for all of them.
   void ReclaimLocked(hipStream_t stream, bool wait);
   void InsertFree(size_t off, size_t len);
   void EraseFree(size_t off, size_t len);

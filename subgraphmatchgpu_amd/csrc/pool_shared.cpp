@@ -9,15 +9,17 @@ void DevicePool::SetShared(bool on) {
   shared_ = on;
 }
 
+// A deferred block goes back to the free list only once its free event has
+// completed, whichever stream asks: the free list is shared, so a block
+// released early for its own stream could be handed to another stream while
+// kernels of the freeing stream still read it.
 void DevicePool::ReclaimLocked(hipStream_t stream, bool wait) {
+  (void)stream;
   size_t keep = 0;
   for (size_t i = 0; i < deferred_.size(); ++i) {
     Deferred& d = deferred_[i];
-    bool done = d.stream == stream && stream != nullptr;
-    if (!done) {
-      if (wait) (void)hipEventSynchronize(d.done);
-      done = wait || hipEventQuery(d.done) == hipSuccess;
-    }
+    if (wait) (void)hipEventSynchronize(d.done);
+    const bool done = wait || hipEventQuery(d.done) == hipSuccess;
     if (done) {
       FreeLocked(d.p);
       spare_events_.push_back(d.done);
@@ -31,7 +33,7 @@ void DevicePool::ReclaimLocked(hipStream_t stream, bool wait) {
 void* DevicePool::Alloc(size_t bytes, hipStream_t stream) {
   std::lock_guard<std::mutex> lk(mu_);
   if (!shared_ || deferred_.empty()) return AllocLocked(bytes);
-  // blocks this stream freed itself are reusable at once
+  // blocks whose frees every stream has passed
   ReclaimLocked(stream, false);
   void* p = AllocLocked(bytes);
   if (!p && !deferred_.empty()) {

@@ -103,6 +103,23 @@ int main(int argc, char** argv) {
     std::fprintf(stderr, "upload: %s\n", sme_last_error(ctx));
     return 1;
   }
+  // LIST=<cap> lists up to cap matches: "total=<t> rows=<r> list=<flattened rows>"
+  if (const char* env = std::getenv("LIST")) {
+    const uint64_t cap = std::strtoull(env, nullptr, 10);
+    const int nv = sme_plan_vertices(plan);
+    std::vector<uint32_t> out(cap * nv + 1);
+    uint64_t total = 0, rows = 0;
+    const int rc = sme_list_matches(ctx, g, plan, vb, static_cast<uint32_t>(ve), out.data(), cap, &total, &rows);
+    if (rc != SME_OK) {
+      std::fprintf(stderr, "list: rc=%d %s\n", rc, sme_last_error(ctx));
+      return rc == SME_ERR_UNSUPPORTED ? 3 : 1;
+    }
+    std::printf("total=%llu rows=%llu list=", static_cast<unsigned long long>(total),
+                static_cast<unsigned long long>(rows));
+    for (uint64_t i = 0; i < rows * nv; ++i) std::printf(i ? ",%u" : "%u", out[i]);
+    std::printf("\n");
+    return 0;
+  }
   uint64_t count = 0, levels[SME_MAX_LEVELS] = {0};
   sme_stats st{};
   // BUCKETS="b0,b1,...,bk" switches to sme_count_buckets over those bounds

@@ -281,7 +281,10 @@ hipError_t hipEventDestroy(hipEvent_t e) {
 }
 hipError_t hipEventCreateWithFlags(hipEvent_t* e, unsigned) { return hipEventCreate(e); }
 // work is synchronous here, so every recorded event has completed
-hipError_t hipEventQuery(hipEvent_t) { return hipSuccess; }
+namespace emu {
+bool g_hold_events = false;
+}
+hipError_t hipEventQuery(hipEvent_t) { return emu::g_hold_events ? hipErrorNotReady : hipSuccess; }
 hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
 
 hipError_t hipEventRecord(hipEvent_t e, hipStream_t) {

@@ -23,7 +23,8 @@ struct dim3 {
 };
 
 typedef int hipError_t;
-enum : int { hipSuccess = 0, hipErrorInvalidValue = 1, hipErrorOutOfMemory = 2, hipErrorNoDevice = 100 };
+enum : int { hipSuccess = 0, hipErrorInvalidValue = 1, hipErrorOutOfMemory = 2, hipErrorNoDevice = 100,
+             hipErrorNotReady = 600 };
 typedef struct EmuStream* hipStream_t;
 typedef struct EmuEvent* hipEvent_t;
 enum hipMemcpyKind {
@@ -68,6 +69,9 @@ hipError_t hipEventElapsedTime(float* ms, hipEvent_t a, hipEvent_t b);
 #define __shared__ static
 
 namespace emu {
+// Test hook: while set, hipEventQuery reports recorded events as not yet
+// complete (models kernels still running on another stream).
+extern bool g_hold_events;
 dim3 CurThread();
 extern dim3 g_block_idx, g_block_dim, g_grid_dim;
 void Launch(dim3 grid, dim3 block, const std::function<void()>& body);

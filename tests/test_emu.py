@@ -17,7 +17,7 @@ EMU = os.path.join(EMU_DIR, "emu_driver")
 
 @pytest.fixture(scope="module")
 def emu():
-    subprocess.run(["make", "-C", EMU_DIR, "-s"], check=True)
+    subprocess.run(["make", "-C", EMU_DIR, "-s", "all"], check=True)
     return EMU
 
 
@@ -213,3 +213,38 @@ def test_emu_server_requests(emu, graphs, merge):
     passes = int(r.stdout.split()[1].split("=")[1])
     nonempty = sum(b > a for _, a, b in reqs)
     assert passes < nonempty if merge == "1" else passes == nonempty
+
+
+def test_emu_match_lists(emu, graphs):
+    """GPSM match lists (capped and whole) equal the oracle's rows and
+    order; a LIGHT plan is rejected as unsupported."""
+    import numpy as np
+
+    g, p = graphs["rmat7"]
+    for q in (0, 1, 2, 3, 4, 7):
+        for algo in (4, 5):
+            for cap in (1 << 20, 17):
+                env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0", LIST=str(cap))
+                r = subprocess.run([emu, p, str(algo), str(q), "3", "1", "5", "100", str(1 << 30)],
+                                   capture_output=True, text=True, env=env, timeout=600)
+                assert r.returncode == 0, r.stderr[-2000:]
+                parts = r.stdout.split()
+                total, rows = int(parts[0].split("=")[1]), int(parts[1].split("=")[1])
+                vals = parts[2].split("=")[1]
+                got = [int(x) for x in vals.split(",")] if vals else []
+                want, wt = oracle.gpsm_list(g.row_ptrs, g.cols, q, 5, 100, cap)
+                assert total == wt and rows == len(want)
+                assert got == want.reshape(-1).tolist(), (q, algo, cap)
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0", LIST="5")
+    r = subprocess.run([emu, p, "6", "0", "3", "1", "0", "10", str(1 << 30)], capture_output=True, text=True,
+                       env=env, timeout=600)
+    assert r.returncode == 3
+
+
+def test_pool_stream_ordered_reuse(emu):
+    """Shared pool: a block freed by one stream stays out of the free list
+    until its free event completes, even after the freeing stream allocates
+    again (otherwise another stream could overwrite data still being read)."""
+    r = subprocess.run([os.path.join(EMU_DIR, "pool_test")], capture_output=True, text=True,
+                       env=dict(os.environ, ASAN_OPTIONS="detect_leaks=0"), timeout=120)
+    assert r.returncode == 0 and r.stdout.strip() == "ok", r.stdout + r.stderr[-2000:]

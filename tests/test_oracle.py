@@ -176,3 +176,41 @@ def test_planted_mistake_is_caught():
     g = clique(6)
     assert oracle.wcoj_count(g.row_ptrs, g.cols, 0, enable_ordering=False) == 6 * kn_closed_form(0, 6)
     assert oracle.wcoj_count(g.row_ptrs, g.cols, 0, enable_ordering=False) != kn_closed_form(0, 6)
+
+
+# Pattern edges of the listed queries (src/query/common/Query.h:84-282).
+PATTERN_EDGES = {
+    0: [(0, 1), (0, 2), (1, 2)],
+    1: [(0, 1), (1, 2), (2, 3), (0, 3)],
+    2: [(0, 1), (0, 2), (0, 3), (1, 2), (2, 3)],
+    3: [(a, b) for a in range(4) for b in range(a + 1, 4)],
+    4: [(0, 1), (0, 2), (1, 2), (1, 3), (2, 4), (3, 4)],
+    7: [(a, b) for a in range(5) for b in range(a + 1, 5)],
+}
+
+
+def check_match_list(g, q, rows, total):
+    """rows are valid embeddings, one per instance (distinct edge images),
+    lexicographically sorted, and as many as the pinned total."""
+    rp, cols = g.row_ptrs, g.cols
+    adj = [set(cols[rp[u]:rp[u + 1]].tolist()) for u in range(g.n)]
+    images = set()
+    for r in rows.tolist():
+        assert len(set(r)) == len(r)
+        for a, b in PATTERN_EDGES[q]:
+            assert r[b] in adj[r[a]], (q, r)
+        images.add(frozenset(frozenset((r[a], r[b])) for a, b in PATTERN_EDGES[q]))
+    assert len(images) == len(rows) == total
+    assert rows.tolist() == sorted(rows.tolist())
+
+
+@pytest.mark.parametrize("q", sorted(PATTERN_EDGES))
+def test_gpsm_list_pinned(fixture_graphs, q):
+    g = fixture_graphs["er300" if q == 4 else "rmat8"]
+    vb, ve = 0, g.n
+    rows, total = oracle.gpsm_list(g.row_ptrs, g.cols, q, vb, ve)
+    assert total == oracle.gpsm_count(g.row_ptrs, g.cols, q, v_begin=vb, v_end=ve)[0]
+    assert len(rows) == total > 0
+    check_match_list(g, q, rows, total)
+    part, t2 = oracle.gpsm_list(g.row_ptrs, g.cols, q, vb, ve, cap=5)
+    assert t2 == total and part.tolist() == rows[:5].tolist()
