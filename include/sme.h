@@ -166,7 +166,7 @@ int sme_plan_vertices(const sme_plan* p);
  *   super_chunks   64-candidate chunks a wavefront takes at once; when they
  *                  all belong to one path, that path's lists are loaded once
  *                  and one short list is staged in LDS (0 or 1: chunk by
- *                  chunk, the default).
+ *                  chunk; a new context starts at 4).
  *   batch_bytes    cap on the device bytes one batch of a level may use
  *                  (0: the level's share of the memory budget).
  * Settings apply to every later call on the context; results never depend

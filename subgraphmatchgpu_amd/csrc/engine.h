@@ -89,14 +89,14 @@ class Context {
   //                  paths use the item schedule)
   //   item_grid      workgroup cap of the item kernels (0: default)
   //   super_chunks   chunks per wave in the verify kernels (single-path
-  //                  runs take the staged fast path; 0 or 1: off, the
-  //                  default; 4 is the fastest on R-MAT scale 20)
+  //                  runs take the staged fast path; 0 or 1: off; 4 by
+  //                  default, the fastest on R-MAT scale 20)
   //   batch_bytes    cap on the device bytes of one batch (0: the level's
   //                  share of the pool)
   struct Tuning {
     uint64_t lane_path_max = 0;
     uint32_t item_grid = 0;
-    uint32_t super_chunks = 0;
+    uint32_t super_chunks = 4;
     uint64_t batch_bytes = 0;
   } tuning;
 

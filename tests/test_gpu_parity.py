@@ -317,8 +317,6 @@ def test_server_results_alone_and_in_the_mix(fixture_graphs, slots, merge, hub):
     assert st["peak_reserved"] <= 2 << 30
 
 
-@pytest.mark.skip(reason="kept off the GPU: its first GPU run coincided with a GPU fault that returned no "
-                         "output; the slices pass in the CPU emulation, no cause shown yet")
 def test_config3_rmat22_slices():
     """RMAT scale 22, edge factor 32 (the memory-capped configuration), full
     CSR resident, 5-clique and house on start-vertex slices: counts and
