@@ -213,7 +213,7 @@ int sme_autotune(sme_ctx* ctx, const sme_graph* g, const sme_plan* const* plans,
  *     beside the passes running (estimate = learned bytes per work unit of
  *     the pattern x summed squared degrees of the covered start vertices,
  *     clamped to [min_reserve_bytes, mem_cap_bytes]; a pattern's first pass
- *     reserves the whole cap).
+ *     reserves mem_cap_bytes / slots).
  * merge = 0 with one slot is the plain path: one request at a time, in
  * arrival order, with the whole budget.  Submit and wait may be called from
  * any host thread. */

@@ -182,14 +182,16 @@ void Server::Dispatch() {
       }
     }
     job->work = work;
-    // admission by estimated intermediate bytes
+    // admission by estimated intermediate bytes; a pattern not seen yet
+    // gets one slot's share of the cap
     const double bpu = bytes_per_unit_[job->q];
-    uint64_t reserve = cfg_.mem_cap;
+    uint64_t reserve = cfg_.mem_cap / static_cast<uint64_t>(cfg_.slots);
     if (bpu >= 0) {
       const double est = bpu * static_cast<double>(work);
       reserve = est >= static_cast<double>(cfg_.mem_cap) ? cfg_.mem_cap : static_cast<uint64_t>(est);
     }
     reserve = std::max(reserve, cfg_.min_reserve);
+    if (cfg_.slots == 1) reserve = cfg_.mem_cap;  // one pass at a time: the whole budget
     job->reserve = reserve;
     cv_dispatch_.wait(lk, [&] { return stop_ || reserved_ + reserve <= cfg_.mem_cap; });
     if (stop_) break;

@@ -10,8 +10,9 @@
 //   - admission: a pass starts only when its estimated intermediate bytes
 //     (a per-pattern bytes-per-work-unit ratio learned from finished passes
 //     times the summed squared degrees of the covered start vertices,
-//     clamped to [min_reserve, mem_cap]) fit beside the passes running; the
-//     pass's batch planner then stays within that reservation.
+//     clamped to [min_reserve, mem_cap]; a pattern not seen yet gets
+//     mem_cap / slots) fit beside the passes running; the pass's batch
+//     planner then stays within that reservation.
 // With merge off and one slot this is the plain path: one request at a time
 // in arrival order, the whole budget each (IntraPartTask over the request's
 // range, src/gpu/task/Task.h:23-43).
