@@ -129,7 +129,7 @@ def main():
     ap.add_argument("--workload", choices=("rmat20", "er-batch", "serve"), default="rmat20")
     ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument("--serve-rate", type=float, default=200.0)
-    ap.add_argument("--serve-slots", type=int, default=8)
+    ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-scale", type=int, default=16)
     ap.add_argument("--serve-edge-factor", type=int, default=16)
     ap.add_argument("--plain-limit-s", type=float, default=2.0,

@@ -291,11 +291,14 @@ class Runner {
   uint64_t ListedRows() const { return list_rows_; }
 
   // Per-bucket counts (device arrays owned by the caller).
-  void SetBuckets(const uint32_t* dev_bounds, uint64_t nbuckets, unsigned long long* dev_counts) {
+  void SetBuckets(const uint32_t* dev_bounds, uint64_t nbuckets, unsigned long long* dev_counts,
+                  const uint32_t* dev_table, uint32_t base) {
     bk_.bounds = dev_bounds;
     bk_.nbuckets = nbuckets;
     bk_.counts = dev_counts;
     bk_.start_v = static_cast<uint8_t>(plan_->ops[0].u);
+    bk_.table = dev_table;
+    bk_.base = base;
   }
 
   void Run(uint32_t v_begin, uint32_t v_end, uint64_t* count, uint64_t* levels, RunStats* stats) {
@@ -1279,8 +1282,14 @@ Status RunPlanBuckets(Context* ctx, const DevGraph* g, const EnginePlan* plan, c
     SME_HIP(hipMemcpyAsync(dbounds.get(), bounds, (nbuckets + 1) * sizeof(uint32_t), hipMemcpyHostToDevice,
                            ctx->stream()));
     LaunchFill64(reinterpret_cast<uint64_t*>(dcounts.get()), nbuckets, 0, ctx->stream());
+    // vertex -> bucket table over the covered range (4 bytes per vertex),
+    // when it takes at most a quarter of what the run may use
+    const uint64_t range = bounds[nbuckets] - bounds[0];
+    const bool use_table = nbuckets > 1 && 4 * range <= ctx->RunAvailable() / 4;
+    Buf<uint32_t> table(ctx, use_table ? range : 0);
+    if (use_table) LaunchBucketTable(dbounds.get(), nbuckets, range, table.get(), ctx->stream());
     Runner r(ctx, g, plan);
-    r.SetBuckets(dbounds.get(), nbuckets, dcounts.get());
+    r.SetBuckets(dbounds.get(), nbuckets, dcounts.get(), table.get(), bounds[0]);
     RunStats st;
     r.Run(bounds[0], bounds[nbuckets], count, level_counts, &st);
     SME_HIP(hipMemcpyAsync(counts, dcounts.get(), nbuckets * sizeof(uint64_t), hipMemcpyDeviceToHost,

@@ -83,6 +83,10 @@ struct BucketRef {
   uint64_t nbuckets;
   unsigned long long* counts;
   uint8_t start_v;
+  // bucket of start vertex v is table[v - bounds[0]] (one load instead of
+  // a binary search over bounds); nullptr: search bounds
+  const uint32_t* table;
+  uint32_t base;
 };
 
 // Item kernels.  `offs` / per-segment arrays are already offset to the
@@ -221,6 +225,9 @@ void LaunchBuildHubBits(const uint64_t* rp, const uint32_t* cols, const uint32_t
 // out[p * nv + w] = column w of path p (nv columns, all materialized).
 void LaunchGatherTuples(const ColSet& cs, int nv, uint64_t n, uint32_t* out, hipStream_t stream);
 void LaunchFill32(uint32_t* out, uint64_t n, uint32_t value, hipStream_t stream);
+// table[v - bounds[0]] = bucket of v for v in [bounds[0], bounds[nbuckets]).
+void LaunchBucketTable(const uint32_t* bounds, uint64_t nbuckets, uint64_t range, uint32_t* table,
+                       hipStream_t stream);
 void LaunchIota32(uint32_t* out, uint64_t n, uint32_t base, hipStream_t stream);
 void LaunchFill64(uint64_t* out, uint64_t n, uint64_t value, hipStream_t stream);
 // Greedy batch split over paths [start, npaths): writes up to max_out pairs

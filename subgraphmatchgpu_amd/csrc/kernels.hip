@@ -69,6 +69,7 @@ constexpr u64 kNoBucket = ~0ull;
 
 // Bucket of start vertex v: largest k with bounds[k] <= v.
 __device__ __forceinline__ u64 BucketOf(const BucketRef& b, uint32_t v) {
+  if (b.table) return b.table[v - b.base];
   uint64_t lo = 0, hi = b.nbuckets - 1;
   while (lo < hi) {
     const uint64_t mid = (lo + hi + 1) >> 1;
@@ -752,6 +753,14 @@ __global__ __launch_bounds__(kThreads) void GatherTuplesKernel(ColSet cs, int nv
   }
 }
 
+__global__ __launch_bounds__(kThreads) void BucketTableKernel(BucketRef b, uint64_t range,
+                                                              uint32_t* table) {
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < range;
+       i += GridStride()) {
+    table[i] = static_cast<uint32_t>(BucketOf(b, b.bounds[0] + static_cast<uint32_t>(i)));
+  }
+}
+
 __global__ __launch_bounds__(kThreads) void BuildHubBitsKernel(const uint64_t* rp, const uint32_t* cols,
                                                                const uint32_t* hubs, uint64_t nhub,
                                                                uint64_t words, uint32_t* bits) {
@@ -979,6 +988,16 @@ void LaunchGatherTuples(const ColSet& cs, int nv, uint64_t n, uint32_t* out, hip
   if (n == 0 || nv <= 0) return;
   hipLaunchKernelGGL(GatherTuplesKernel, dim3(GridFor(n * nv, kThreads)), dim3(kThreads), 0, stream, cs, nv,
                      n, out);
+}
+
+void LaunchBucketTable(const uint32_t* bounds, uint64_t nbuckets, uint64_t range, uint32_t* table,
+                       hipStream_t stream) {
+  if (range == 0) return;
+  BucketRef b{};
+  b.bounds = bounds;
+  b.nbuckets = nbuckets;
+  hipLaunchKernelGGL(BucketTableKernel, dim3(GridFor(range, kThreads)), dim3(kThreads), 0, stream, b, range,
+                     table);
 }
 
 void LaunchFill32(uint32_t* out, uint64_t n, uint32_t value, hipStream_t stream) {

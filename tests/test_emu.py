@@ -114,6 +114,30 @@ def test_emu_buckets(emu, graphs):
             assert got == want, (q, algo)
 
 
+def test_emu_buckets_without_table(emu, graphs, tmp_path):
+    """Buckets over a range too large for the vertex->bucket table under a
+    40 KB budget (the rmat7 edges plus isolated vertices up to 20,000): the
+    binary search over the bounds gives the same counts."""
+    import numpy as np
+
+    g0, _ = graphs["rmat7"]
+    rp = np.concatenate([g0.row_ptrs, np.full(20000 - g0.n, g0.row_ptrs[-1], dtype=np.uint64)])
+    g = sme.Graph(rp, g0.cols.copy())
+    p = str(tmp_path / "padded.bin")
+    sme.write_bin(p, g)
+    bounds = [0, 3, 17, 40, 41, 90, 128, 5000, 20000]
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0", BUCKETS=",".join(map(str, bounds)))
+    for q in (0, 4):
+        for algo in (6, 4):
+            r = subprocess.run([emu, p, str(algo), str(q), "3", "1", "0", str(g.n), "40000"],
+                               capture_output=True, text=True, env=env, timeout=600)
+            assert r.returncode == 0, r.stderr[-2000:]
+            got = [int(x) for x in r.stdout.split("buckets=")[1].split(",")]
+            count = oracle.light_count if algo == 6 else oracle.gpsm_count
+            assert got == [count(g.row_ptrs, g.cols, q, v_begin=a, v_end=b)[0]
+                           for a, b in zip(bounds[:-1], bounds[1:])], (q, algo)
+
+
 def test_emu_clique_without_ordering(emu, graphs):
     g, p = graphs["k7"]
     for q in (0, 3, 7):
@@ -141,7 +165,7 @@ def test_emu_er_batch_members(emu, tmp_path):
         assert any(want)
 
 
-@pytest.mark.parametrize("tune", ["4,0,0", "64,3,0", "1000000,1,20000"])
+@pytest.mark.parametrize("tune", ["4,0,0", "64,3,0", "1000000,1,20000", "0,8,0,2", "16,16,0,4"])
 def test_emu_tunings_do_not_change_results(emu, graphs, tune):
     """Lane-per-path counting, tiny grids and tiny batch caps: same counts,
     levels and per-bucket counts."""
@@ -172,7 +196,8 @@ def test_emu_super_chunk_fast_path(emu, tmp_path):
     from conftest import clique
 
     cases = [(sme.gen_rmat(9, 32, seed=5), ((0, "0,0,0,2", None), (0, "0,0,0,4", "16"),
-                                            (2, "0,0,0,4", "16"), (3, "0,0,0,16", None))),
+                                            (2, "0,0,0,4", "16"), (3, "0,0,0,16", None),
+                                            (3, "0,8,0,2", None), (1, "64,16,0,4", "16"))),
              (clique(200), ((0, "0,0,0,4", "16"),))]
     for i, (g, runs) in enumerate(cases):
         p = str(tmp_path / f"g{i}.bin")
