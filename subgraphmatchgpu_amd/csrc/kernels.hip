@@ -81,11 +81,35 @@ __device__ __forceinline__ u64 BucketOf(const BucketRef& b, uint32_t v) {
   return lo;
 }
 
-// Segmented wave sum over lanes with equal keys (keys are non-decreasing
-// across the wave); the last lane of each run adds the run's total.  Must
-// be reached by all 64 lanes.
-__device__ __forceinline__ void BucketAdd(const BucketRef& b, u64 key, u64 v) {
+// Per-wave running bucket sum (wave-uniform).  Bucket counts are added
+// into it while the wave's keys stay on one bucket and written with one
+// atomic when the bucket changes, so a wave issues about one atomic per
+// bucket it touches instead of one per chunk (a few buckets would otherwise
+// serialize every wave on the same counters).
+struct BucketAcc {
+  u64 key;
+  u64 sum;
+};
+
+__device__ __forceinline__ void BucketFlush(const BucketRef& b, BucketAcc* acc) {
+  if ((threadIdx.x & (kWave - 1)) == 0 && acc->sum && acc->key != kNoBucket)
+    atomicAdd(b.counts + acc->key, acc->sum);
+  acc->key = kNoBucket;
+  acc->sum = 0;
+}
+
+// Adds each lane's value under its bucket key (keys non-decreasing across
+// the wave; kNoBucket for lanes with nothing).  Must be reached by all 64
+// lanes.  Runs before the wave's last bucket are written at once; the last
+// one stays in the accumulator.
+__device__ __forceinline__ void BucketAccAdd(const BucketRef& b, BucketAcc* acc, u64 key, u64 val) {
   const int lane = threadIdx.x & (kWave - 1);
+  if (__ballot(key != kNoBucket && key != acc->key) == 0) {
+    acc->sum += WaveSum(val);
+    return;
+  }
+  BucketFlush(b, acc);
+  u64 v = val;
 #pragma unroll
   for (int o = 1; o < kWave; o <<= 1) {
     const u64 y = __shfl_up(v, o);
@@ -94,7 +118,13 @@ __device__ __forceinline__ void BucketAdd(const BucketRef& b, u64 key, u64 v) {
   }
   const u64 next = __shfl_down(key, 1);
   const bool run_end = lane == kWave - 1 || next != key;
-  if (run_end && v && key != kNoBucket) atomicAdd(b.counts + key, v);
+  const u64 live = __ballot(key != kNoBucket);  // not 0: some key differed from acc
+  const int last = kWave - 1 - __clzll(static_cast<long long>(live));
+  const u64 last_key = __shfl(key, last);
+  const u64 last_sum = __shfl(v, last);
+  if (run_end && key != kNoBucket && key != last_key && v) atomicAdd(b.counts + key, v);
+  acc->key = last_key;
+  acc->sum = last_sum;
 }
 
 // ThreadCountWithComputedVertex (CheckConstraintsCommon.cuh:131-155): how
@@ -253,6 +283,7 @@ __device__ uint64_t DirectCount(const uint64_t* c, int q) {
 __global__ __launch_bounds__(kThreads) void CountPathsKernel(CountPathArgs a) {
   u64 local = 0;
   const int lane = threadIdx.x & (kWave - 1);
+  BucketAcc bacc{kNoBucket, 0};
   // wave-uniform trip count, so per-bucket sums can use the whole wave
   for (uint64_t base = blockIdx.x * static_cast<uint64_t>(blockDim.x) + (threadIdx.x - lane);
        base < a.npaths; base += GridStride()) {
@@ -273,9 +304,10 @@ __global__ __launch_bounds__(kThreads) void CountPathsKernel(CountPathArgs a) {
     local += val;
     if (a.bk.bounds) {
       const u64 key = p < a.npaths ? BucketOf(a.bk, Gather(a.cs, a.bk.start_v, p)) : kNoBucket;
-      BucketAdd(a.bk, key, val);
+      BucketAccAdd(a.bk, &bacc, key, val);
     }
   }
+  if (a.bk.bounds) BucketFlush(a.bk, &bacc);
   local = WaveSum(local);
   if ((threadIdx.x & (kWave - 1)) == 0 && local) {
     if (a.acc) atomicAdd(a.acc, local);
@@ -337,7 +369,7 @@ __device__ __forceinline__ const uint32_t* ItemSrc(const ItemArgs& a, uint64_t s
 // segments).  Returns the lane's count contribution (sum modes) or writes
 // the chunk's mask and popcount (mask mode).
 template <int kMode>
-__device__ __forceinline__ u64 GenericChunk(const ItemArgs& a, uint64_t c, int lane) {
+__device__ __forceinline__ u64 GenericChunk(const ItemArgs& a, uint64_t c, int lane, BucketAcc* bacc) {
   const uint64_t t = c * kWave + lane;
   bool ok = t < a.nitems;
   uint64_t s = 0;
@@ -382,7 +414,7 @@ __device__ __forceinline__ u64 GenericChunk(const ItemArgs& a, uint64_t c, int l
     }
   } else if (a.bk.bounds) {
     const u64 key = t < a.nitems ? BucketOf(a.bk, Gather(a.cs, a.bk.start_v, s)) : kNoBucket;
-    BucketAdd(a.bk, key, val);
+    BucketAccAdd(a.bk, bacc, key, val);
   }
   return val;
 }
@@ -544,6 +576,7 @@ __global__ __launch_bounds__(kThreads) void VerifyKernel(ItemArgs a) {
   const uint64_t M = a.super_chunks > 1 ? a.super_chunks : 1;
   const uint64_t nsuper = (a.nchunks + M - 1) / M;
   u64 local = 0;
+  BucketAcc bacc{kNoBucket, 0};
   for (uint64_t sc = wave; sc < nsuper; sc += nwaves) {
     const uint64_t c0 = sc * M;
     const uint64_t c1 = c0 + M < a.nchunks ? c0 + M : a.nchunks;
@@ -555,16 +588,15 @@ __global__ __launch_bounds__(kThreads) void VerifyKernel(ItemArgs a) {
       fast = SegmentOf(a, c1 - 1, t_last) == s;
     }
     if (fast) {
-      u64 seg = FastSegment<kMode>(a, s, c0, c1, stage, lane);
+      const u64 seg = FastSegment<kMode>(a, s, c0, c1, stage, lane);
       local += seg;
-      if (kMode != kModeMask && a.bk.bounds) {
-        seg = WaveSum(seg);
-        if (lane == 0 && seg) atomicAdd(a.bk.counts + BucketOf(a.bk, Gather(a.cs, a.bk.start_v, s)), seg);
-      }
+      if (kMode != kModeMask && a.bk.bounds)
+        BucketAccAdd(a.bk, &bacc, BucketOf(a.bk, Gather(a.cs, a.bk.start_v, s)), seg);
     } else {
-      for (uint64_t c = c0; c < c1; ++c) local += GenericChunk<kMode>(a, c, lane);
+      for (uint64_t c = c0; c < c1; ++c) local += GenericChunk<kMode>(a, c, lane, &bacc);
     }
   }
+  if (kMode != kModeMask && a.bk.bounds) BucketFlush(a.bk, &bacc);
   if (kMode != kModeMask) {
     local = WaveSum(local);
     if (lane == 0 && local) {
@@ -650,6 +682,7 @@ __global__ __launch_bounds__(kThreads) void PrepCountKernel(PrepCountArgs a) {
   const int lane = threadIdx.x & (kWave - 1);
   const uint64_t np = a.pa.npaths;
   u64 local = 0, nlane = 0;
+  BucketAcc bacc{kNoBucket, 0};
   for (uint64_t base = blockIdx.x * static_cast<uint64_t>(blockDim.x) + (threadIdx.x - lane);
        base < np; base += GridStride()) {
     const uint64_t p = base + lane;
@@ -671,9 +704,10 @@ __global__ __launch_bounds__(kThreads) void PrepCountKernel(PrepCountArgs a) {
     local += val;
     if (a.it.bk.bounds) {
       const u64 key = p < np ? BucketOf(a.it.bk, Gather(a.it.cs, a.it.bk.start_v, p)) : kNoBucket;
-      BucketAdd(a.it.bk, key, val);
+      BucketAccAdd(a.it.bk, &bacc, key, val);
     }
   }
+  if (a.it.bk.bounds) BucketFlush(a.it.bk, &bacc);
   local = WaveSum(local);
   nlane = WaveSum(nlane);
   if (lane == 0) {

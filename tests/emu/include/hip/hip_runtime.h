@@ -121,6 +121,12 @@ inline void __builtin_amdgcn_wave_barrier() { (void)emu::WaveExchange(emu::kBall
 inline unsigned int __builtin_amdgcn_readfirstlane(unsigned int v) {
   return static_cast<unsigned int>(emu::WaveExchange(emu::kReadLane, v, 0));
 }
+template <typename T>
+inline T __shfl(T v, int src, int width = 64) {
+  (void)width;
+  return emu::FromBits<T>(emu::WaveExchange(emu::kReadLane, emu::ToBits(v), static_cast<unsigned>(src)));
+}
+inline int __clzll(long long x) { return x ? __builtin_clzll(static_cast<unsigned long long>(x)) : 64; }
 inline unsigned int __popcll(unsigned long long x) { return static_cast<unsigned>(__builtin_popcountll(x)); }
 inline unsigned long long __lanemask_lt() {
   const unsigned lane = threadIdx.x & 63u;
