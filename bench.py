@@ -139,6 +139,8 @@ def main():
     ap.add_argument("--per-graph-sample", type=int, default=1000,
                     help="members timed with one pass each (er-batch)")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--autotune", action="store_true",
+                    help="rmat20: let the autotuner pick the schedule before timing")
     ap.add_argument("--tune", default="0,0,0,4",
                     help="lane_path_max,item_grid,batch_bytes,super_chunks schedule setting (rmat20)")
     ap.add_argument("--summarize-pmc", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "CALIB_MIB"),
@@ -178,6 +180,10 @@ def main():
     if args.tune:
         f = [int(x) for x in args.tune.split(",")] + [0, 0, 0, 0]
         dev.set_tuning(lane_path_max=f[0], item_grid=f[1], batch_bytes=f[2], super_chunks=f[3])
+    if args.autotune:
+        t = time.perf_counter()
+        best, ms, trials = dev.autotune(dg, plans, sample_fraction=0.02)
+        log(f"autotune: {best} ({time.perf_counter() - t:.1f}s, {len(trials)} trials)")
     log(f"schedule: {dev.get_tuning()}")
 
     def step():
