@@ -142,7 +142,7 @@ def main():
     ap.add_argument("--autotune", action="store_true",
                     help="rmat20: let the autotuner pick the schedule before timing")
     ap.add_argument("--tune", default="0,0,0,4",
-                    help="lane_path_max,item_grid,batch_bytes,super_chunks schedule setting (rmat20)")
+                    help="lane_path_max,item_grid,batch_bytes,super_chunks[,stream_ratio] schedule (rmat20)")
     ap.add_argument("--summarize-pmc", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "CALIB_MIB"),
                     help="print HBM traffic per intersect launch from two rocprofv3 --pmc runs")
     args = ap.parse_args()
@@ -178,8 +178,9 @@ def main():
     log(f"upload: {1e3 * (time.perf_counter() - t0):.1f} ms, {dg.hubs} hub bitmaps")
     plans = [sme.Plan(q, sme.GPU_LIGHT, sme.CL_SPECIAL) for q in QUERIES]
     if args.tune:
-        f = [int(x) for x in args.tune.split(",")] + [0, 0, 0, 0]
-        dev.set_tuning(lane_path_max=f[0], item_grid=f[1], batch_bytes=f[2], super_chunks=f[3])
+        f = [int(x) for x in args.tune.split(",")] + [0, 0, 0, 0, 0]
+        dev.set_tuning(lane_path_max=f[0], item_grid=f[1], batch_bytes=f[2], super_chunks=f[3],
+                       stream_ratio=f[4])
     if args.autotune:
         t = time.perf_counter()
         best, ms, trials = dev.autotune(dg, plans, sample_fraction=0.02)

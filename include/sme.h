@@ -169,6 +169,12 @@ int sme_plan_vertices(const sme_plan* p);
  *                  chunk; a new context starts at 4).
  *   batch_bytes    cap on the device bytes one batch of a level may use
  *                  (0: the level's share of the memory budget).
+ *   stream_ratio   pivot rule when a level extends a reused candidate
+ *                  segment with one adjacency list: if the segment holds at
+ *                  most 512 candidates and the list at most stream_ratio
+ *                  times as many, the list is read in order as the pivot and
+ *                  the segment is probed (from LDS) instead of probing the
+ *                  list at random (0: the shorter is the pivot).
  * Settings apply to every later call on the context; results never depend
  * on them. */
 typedef struct {
@@ -176,6 +182,8 @@ typedef struct {
   uint32_t item_grid;
   uint32_t super_chunks;
   uint64_t batch_bytes;
+  uint32_t stream_ratio;
+  uint32_t reserved;
 } sme_tuning;
 int sme_set_tuning(sme_ctx* ctx, const sme_tuning* t);
 int sme_get_tuning(const sme_ctx* ctx, sme_tuning* t);
@@ -187,7 +195,7 @@ typedef struct {
 
 /* Autotuner: times the plans on a sample of the start vertices for each
  * candidate setting (coordinate descent over lane_path_max, super_chunks,
- * item_grid and batch_bytes, starting from the context's current setting), installs the
+ * stream_ratio, item_grid and batch_bytes, starting from the context's current setting), installs the
  * fastest on the context and returns it.  With bounds (nbuckets >= 1) the
  * sample is 8 evenly spaced blocks of consecutive buckets covering about
  * sample_fraction of the buckets, run through sme_count_buckets; without

@@ -54,11 +54,13 @@ class _Stats(ctypes.Structure):
 
 class _Tuning(ctypes.Structure):
     _fields_ = [("lane_path_max", ctypes.c_uint64), ("item_grid", ctypes.c_uint32),
-                ("super_chunks", ctypes.c_uint32), ("batch_bytes", ctypes.c_uint64)]
+                ("super_chunks", ctypes.c_uint32), ("batch_bytes", ctypes.c_uint64),
+                ("stream_ratio", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
 
     def as_dict(self) -> dict:
         return {"lane_path_max": int(self.lane_path_max), "item_grid": int(self.item_grid),
-                "super_chunks": int(self.super_chunks), "batch_bytes": int(self.batch_bytes)}
+                "super_chunks": int(self.super_chunks), "batch_bytes": int(self.batch_bytes),
+                "stream_ratio": int(self.stream_ratio)}
 
 
 class _Trial(ctypes.Structure):
@@ -381,15 +383,17 @@ class Device:
         return out[:rows.value], int(total.value)
 
     def set_tuning(self, lane_path_max: int = 0, item_grid: int = 0, batch_bytes: int = 0,
-                   super_chunks: int = 0):
+                   super_chunks: int = 0, stream_ratio: int = 0):
         """Schedule tunables for later calls (results never depend on them):
         lane_path_max -- counting-level paths with at most this many pivot
         candidates are verified by one lane each; item_grid -- workgroup cap
         of the candidate kernels (0: default); batch_bytes -- cap on one
         batch's device bytes (0: the level's share of the budget);
         super_chunks -- 64-candidate chunks per wavefront step (runs inside
-        one path load the path once and stage a list window in LDS)."""
-        t = _Tuning(lane_path_max, item_grid, super_chunks, batch_bytes)
+        one path load the path once and stage a list window in LDS);
+        stream_ratio -- read an adjacency list in order as the pivot when it
+        is at most this many times a short reused segment (0: off)."""
+        t = _Tuning(lane_path_max, item_grid, super_chunks, batch_bytes, stream_ratio, 0)
         _check(library().sme_set_tuning(self._h, ctypes.byref(t)), self._h)
 
     def get_tuning(self) -> dict:
@@ -480,7 +484,8 @@ class Server:
         cfg = _ServerConfig(mem_cap_bytes, slots, int(merge), max_merge, min_reserve_bytes,
                             hub_min_degree, hub_max_bytes,
                             _Tuning(t.get("lane_path_max", 0), t.get("item_grid", 0),
-                                    t.get("super_chunks", 0), t.get("batch_bytes", 0)))
+                                    t.get("super_chunks", 0), t.get("batch_bytes", 0),
+                                    t.get("stream_ratio", 0), 0))
         self._rp = np.ascontiguousarray(g.row_ptrs, dtype=np.uint64)
         self._cols = np.ascontiguousarray(g.cols, dtype=np.uint32)
         h = ctypes.c_void_p()

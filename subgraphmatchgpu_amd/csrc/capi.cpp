@@ -224,6 +224,7 @@ int sme_set_tuning(sme_ctx* ctx, const sme_tuning* t) {
   ctx->impl.tuning.item_grid = t->item_grid;
   ctx->impl.tuning.super_chunks = t->super_chunks;
   ctx->impl.tuning.batch_bytes = t->batch_bytes;
+  ctx->impl.tuning.stream_ratio = t->stream_ratio;
   return SME_OK;
 }
 
@@ -234,6 +235,7 @@ int sme_get_tuning(const sme_ctx* ctx, sme_tuning* t) {
   t->item_grid = ctx->impl.tuning.item_grid;
   t->super_chunks = ctx->impl.tuning.super_chunks;
   t->batch_bytes = ctx->impl.tuning.batch_bytes;
+  t->stream_ratio = ctx->impl.tuning.stream_ratio;
   return SME_OK;
 }
 
@@ -289,7 +291,8 @@ int sme_autotune(sme_ctx* ctx, const sme_graph* g, const sme_plan* const* plans,
   int n = 0;
   auto record = [&](const sme::Context::Tuning& t, double ms) {
     if (trials && n < cap) {
-      trials[n].tuning = sme_tuning{t.lane_path_max, t.item_grid, t.super_chunks, t.batch_bytes};
+      trials[n].tuning = sme_tuning{t.lane_path_max, t.item_grid, t.super_chunks, t.batch_bytes,
+                                    t.stream_ratio, 0};
       trials[n].ms = ms;
     }
     ++n;
@@ -313,7 +316,8 @@ int sme_autotune(sme_ctx* ctx, const sme_graph* g, const sme_plan* const* plans,
   const uint32_t super_opts[] = {0, 2, 4, 8, 16};
   const uint32_t grid_opts[] = {0, 8192, 4096, 2048, 1024};
   const uint64_t batch_opts[] = {0, 4ull << 30, 1ull << 30, 256ull << 20, 64ull << 20};
-  for (int dim = 0; dim < 4; ++dim) {
+  const uint32_t stream_opts[] = {0, 4, 8, 16, 32};
+  for (int dim = 0; dim < 5; ++dim) {
     const int nopt = 5;
     sme::Context::Tuning dim_best = cur;
     double dim_ms = cur_ms;
@@ -323,8 +327,10 @@ int sme_autotune(sme_ctx* ctx, const sme_graph* g, const sme_plan* const* plans,
       if (dim == 1) t.super_chunks = super_opts[k];
       if (dim == 2) t.item_grid = grid_opts[k];
       if (dim == 3) t.batch_bytes = batch_opts[k];
+      if (dim == 4) t.stream_ratio = stream_opts[k];
       if (t.lane_path_max == cur.lane_path_max && t.item_grid == cur.item_grid &&
-          t.super_chunks == cur.super_chunks && t.batch_bytes == cur.batch_bytes)
+          t.super_chunks == cur.super_chunks && t.batch_bytes == cur.batch_bytes &&
+          t.stream_ratio == cur.stream_ratio)
         continue;
       double t_ms = 0;
       rc = measure(t, &t_ms);
@@ -342,7 +348,8 @@ int sme_autotune(sme_ctx* ctx, const sme_graph* g, const sme_plan* const* plans,
     cur_ms = dim_ms;
   }
   c.tuning = cur;
-  *best = sme_tuning{cur.lane_path_max, cur.item_grid, cur.super_chunks, cur.batch_bytes};
+  *best = sme_tuning{cur.lane_path_max, cur.item_grid, cur.super_chunks, cur.batch_bytes,
+                     cur.stream_ratio, 0};
   if (best_ms) *best_ms = cur_ms;
   if (ntrials) *ntrials = n;
   return SME_OK;
@@ -366,6 +373,7 @@ int sme_server_open(int device, const uint64_t* row_ptrs, const uint32_t* cols, 
   c.tuning.item_grid = cfg->tuning.item_grid;
   c.tuning.super_chunks = cfg->tuning.super_chunks;
   c.tuning.batch_bytes = cfg->tuning.batch_bytes;
+  c.tuning.stream_ratio = cfg->tuning.stream_ratio;
   std::string err;
   const sme::Status st = s->impl.Open(device, row_ptrs, cols, n, nnz, c, &err);
   if (st != sme::kOk) {

@@ -93,11 +93,14 @@ class Context {
   //                  default, the fastest on R-MAT scale 20)
   //   batch_bytes    cap on the device bytes of one batch (0: the level's
   //                  share of the pool)
+  //   stream_ratio   pivot rule for a reused segment plus one adjacency
+  //                  list (see PrepComputeArgs; 0: shortest list)
   struct Tuning {
     uint64_t lane_path_max = 0;
     uint32_t item_grid = 0;
     uint32_t super_chunks = 4;
     uint64_t batch_bytes = 0;
+    uint32_t stream_ratio = 0;
   } tuning;
 
   // profiling

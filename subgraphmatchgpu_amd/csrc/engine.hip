@@ -689,6 +689,7 @@ class Runner {
     pa.size = size.get();
     pa.pstart = pstart.get();
     pa.piv = piv.get();
+    pa.stream_ratio = ctx_->tuning.stream_ratio;
 
     uint32_t bmask = 0;
     for (int i = 0; i < op.nb; ++i) bmask |= 1u << op.bconn[i];

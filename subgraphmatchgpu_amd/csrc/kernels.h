@@ -59,7 +59,15 @@ struct PrepComputeArgs {
   uint64_t* size;    // out: items of the path (restricted pivot length)
   uint64_t* pstart;  // out: first item's index into cols
   uint8_t* piv;      // out: which backward list is the pivot
+  // Streaming pivot: with a reused segment of at most kStageEntries values
+  // and one adjacency list of length <= stream_ratio x that segment, the
+  // adjacency list becomes the pivot (read in order) and the short segment
+  // is what gets probed (staged in LDS by the verify fast path); 0: the
+  // shortest list is the pivot.
+  uint32_t stream_ratio;
 };
+
+constexpr uint32_t kStageEntries = 512;  // LDS entries per wave for a staged list
 
 // Per-path set-up of a scan over an existing candidate set (MATERIALIZE,
 // FILTER_COMPUTE): the segment is cut to the < / > window.

@@ -13,7 +13,7 @@ constexpr int kThreads = 256;
 constexpr int kWave = 64;
 constexpr int kWavesPerBlock = kThreads / kWave;
 constexpr uint64_t kMaxGrid = 16384;
-constexpr uint32_t kStageCap = 512;  // LDS entries per wave for a staged list window
+constexpr uint32_t kStageCap = kStageEntries;
 
 using u64 = unsigned long long;
 
@@ -192,6 +192,19 @@ __device__ __forceinline__ void PrepPath(const PrepComputeArgs& a, uint64_t p, u
       best_b = b;
       best_e = e;
       have = true;
+      windowed = false;
+    }
+  }
+  if (a.stream_ratio && a.has_reuse && a.nb == 1 && best == a.nb && best_e > best_b &&
+      best_e - best_b <= kStageEntries) {
+    // stream the adjacency list instead when it is at most stream_ratio
+    // times the (windowed) reused segment
+    const uint32_t v = Gather(a.cs, a.bconn[0], p);
+    const uint64_t b = a.rp[v], e = a.rp[v + 1];
+    if (e - b <= static_cast<uint64_t>(a.stream_ratio) * (best_e - best_b)) {
+      best = 0;
+      best_b = b;
+      best_e = e;
       windowed = false;
     }
   }

@@ -82,7 +82,7 @@ int main(int argc, char** argv) {
     return 1;
   }
   if (argc == 10) sme_set_hub_bitmaps(ctx, std::strtoull(argv[9], nullptr, 10), 1ull << 30);
-  // TUNE="lane_path_max,item_grid,batch_bytes[,super_chunks]" sets the
+  // TUNE="lane_path_max,item_grid,batch_bytes[,super_chunks[,stream_ratio]]" sets the
   // schedule tunables
   if (const char* env = std::getenv("TUNE")) {
     sme_tuning t{};
@@ -91,6 +91,7 @@ int main(int argc, char** argv) {
     if (*s == ',') t.item_grid = static_cast<uint32_t>(std::strtoul(s + 1, &s, 10));
     if (*s == ',') t.batch_bytes = std::strtoull(s + 1, &s, 10);
     if (*s == ',') t.super_chunks = static_cast<uint32_t>(std::strtoul(s + 1, &s, 10));
+    if (*s == ',') t.stream_ratio = static_cast<uint32_t>(std::strtoul(s + 1, &s, 10));
     sme_set_tuning(ctx, &t);
   }
   sme_plan* plan = nullptr;

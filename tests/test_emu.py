@@ -165,7 +165,8 @@ def test_emu_er_batch_members(emu, tmp_path):
         assert any(want)
 
 
-@pytest.mark.parametrize("tune", ["4,0,0", "64,3,0", "1000000,1,20000", "0,8,0,2", "16,16,0,4"])
+@pytest.mark.parametrize("tune", ["4,0,0", "64,3,0", "1000000,1,20000", "0,8,0,2", "16,16,0,4",
+                                  "0,0,0,4,8", "8,0,0,2,1000"])
 def test_emu_tunings_do_not_change_results(emu, graphs, tune):
     """Lane-per-path counting, tiny grids and tiny batch caps: same counts,
     levels and per-bucket counts."""
@@ -197,7 +198,9 @@ def test_emu_super_chunk_fast_path(emu, tmp_path):
 
     cases = [(sme.gen_rmat(9, 32, seed=5), ((0, "0,0,0,2", None), (0, "0,0,0,4", "16"),
                                             (2, "0,0,0,4", "16"), (3, "0,0,0,16", None),
-                                            (3, "0,8,0,2", None), (1, "64,16,0,4", "16"))),
+                                            (3, "0,8,0,2", None), (1, "64,16,0,4", "16"),
+                                            (3, "0,0,0,4,16", "16"), (7, "0,0,0,2,1000", None),
+                                            (3, "32,0,0,4,8", None))),
              (clique(200), ((0, "0,0,0,4", "16"),))]
     for i, (g, runs) in enumerate(cases):
         p = str(tmp_path / f"g{i}.bin")

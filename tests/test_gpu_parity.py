@@ -201,7 +201,8 @@ def test_self_loops_handled(dev):
 
 @pytest.mark.parametrize("tune", [dict(lane_path_max=8), dict(lane_path_max=1 << 20, item_grid=64),
                                   dict(item_grid=1, batch_bytes=1 << 16), dict(super_chunks=2),
-                                  dict(super_chunks=16, lane_path_max=8, item_grid=512)])
+                                  dict(super_chunks=16, lane_path_max=8, item_grid=512),
+                                  dict(stream_ratio=16), dict(stream_ratio=1000, lane_path_max=32)])
 def test_tunings_do_not_change_results(fixture_graphs, tune):
     g = fixture_graphs["rmat10"]
     d = sme.Device(0, budget_bytes=2 << 30)
@@ -231,17 +232,17 @@ def light_oracle(name, g, q, cl=3):
     return _ORACLE_CACHE[key]
 
 
-@pytest.mark.parametrize("super_chunks", [2, 4, 16])
-def test_super_chunks_with_hubs(fixture_graphs, super_chunks):
+@pytest.mark.parametrize("super_chunks,stream_ratio", [(2, 0), (4, 0), (16, 0), (4, 8), (2, 1000)])
+def test_super_chunks_with_hubs(fixture_graphs, super_chunks, stream_ratio):
     """Long single-path runs (staged fast path) with and without hub rows."""
     g = fixture_graphs["rmat12"]
     for hub in (0, 32):
         d = sme.Device(0, budget_bytes=2 << 30)
         d.set_hub_bitmaps(hub, 1 << 30)
-        d.set_tuning(super_chunks=super_chunks)
+        d.set_tuning(super_chunks=super_chunks, stream_ratio=stream_ratio)
         dg = d.upload(g)
         for q in (0, 1, 2, 3, 4, 7):
-            say(f"case super_chunks={super_chunks} hub={hub} q={q}")
+            say(f"case super_chunks={super_chunks} stream_ratio={stream_ratio} hub={hub} q={q}")
             want = light_oracle("rmat12", g, q)
             r = d.count(dg, sme.Plan(q))
             assert (r.count, r.levels) == want
