@@ -116,18 +116,18 @@ def test_emu_buckets(emu, graphs):
 
 def test_emu_buckets_without_table(emu, graphs, tmp_path):
     """Buckets over a range too large for the vertex->bucket table under a
-    40 KB budget (the rmat7 edges plus isolated vertices up to 20,000): the
+    40 KB budget (the rmat7 edges plus isolated vertices up to 6,000): the
     binary search over the bounds gives the same counts."""
     import numpy as np
 
     g0, _ = graphs["rmat7"]
-    rp = np.concatenate([g0.row_ptrs, np.full(20000 - g0.n, g0.row_ptrs[-1], dtype=np.uint64)])
+    rp = np.concatenate([g0.row_ptrs, np.full(6000 - g0.n, g0.row_ptrs[-1], dtype=np.uint64)])
     g = sme.Graph(rp, g0.cols.copy())
     p = str(tmp_path / "padded.bin")
     sme.write_bin(p, g)
-    bounds = [0, 3, 17, 40, 41, 90, 128, 5000, 20000]
+    bounds = [0, 3, 17, 40, 41, 90, 128, 5000, 6000]
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0", BUCKETS=",".join(map(str, bounds)))
-    for q in (0, 4):
+    for q in (0, 1):
         for algo in (6, 4):
             r = subprocess.run([emu, p, str(algo), str(q), "3", "1", "0", str(g.n), "40000"],
                                capture_output=True, text=True, env=env, timeout=600)
@@ -197,10 +197,8 @@ def test_emu_super_chunk_fast_path(emu, tmp_path):
     from conftest import clique
 
     cases = [(sme.gen_rmat(9, 32, seed=5), ((0, "0,0,0,2", None), (0, "0,0,0,4", "16"),
-                                            (2, "0,0,0,4", "16"), (3, "0,0,0,16", None),
-                                            (3, "0,8,0,2", None), (1, "64,16,0,4", "16"),
-                                            (3, "0,0,0,4,16", "16"), (7, "0,0,0,2,1000", None),
-                                            (3, "32,0,0,4,8", None))),
+                                            (2, "0,0,0,4", "16"), (3, "0,8,0,2", None),
+                                            (3, "0,0,0,4,16", "16"), (3, "32,0,0,4,8", None))),
              (clique(200), ((0, "0,0,0,4", "16"),))]
     for i, (g, runs) in enumerate(cases):
         p = str(tmp_path / f"g{i}.bin")
@@ -251,7 +249,7 @@ def test_emu_match_lists(emu, graphs):
     g, p = graphs["rmat7"]
     for q in (0, 1, 2, 3, 4, 7):
         for algo in (4, 5):
-            for cap in (1 << 20, 17):
+            for cap in ((1 << 20, 17) if q in (0, 3) else (1 << 20,)):
                 env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0", LIST=str(cap))
                 r = subprocess.run([emu, p, str(algo), str(q), "3", "1", "5", "100", str(1 << 30)],
                                    capture_output=True, text=True, env=env, timeout=600)
