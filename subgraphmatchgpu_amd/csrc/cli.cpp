@@ -46,6 +46,8 @@ void Usage() {
   std::printf(
       "./patternmatchgpu -f FILENAME -d IS_DIRECTED -a ALGORITHM -q QUERY -m EXECUTE_MODE "
       "-c COMPRESS_LEVEL -r ENABLE_ORDERING -v DEV_NUM -b DEVICE_BUDGET_GB\n"
+      "-l MATERIALIZE (1: after the result line, print up to -L MAX_MATCHES matches, one per\n"
+      "   line, data vertices of query vertices 0.., lexicographic; GPU_GPSM / GPU_GPSM_REUSE)\n"
       "ALGORITHM: GPU_GPSM=4, GPU_LIGHT=6\n"
       "EXECUTE_MODE: IN_MEMORY_GPU=1 (HYBRID_CPU_GPU=0 is accepted with one partition)\n"
       "QUERY: Q0 (triangle) 0, Q1 (square) 1, Q2 (chordal square) 2, Q3 (4 clique) 3, "
@@ -77,6 +79,8 @@ int main(int argc, char** argv) {
   const int compress = static_cast<int>(a.Int("-c", SME_CL_SPECIAL));
   const int ordering = static_cast<int>(a.Int("-r", 1));
   const double budget_gb = std::atof(a.Str("-b", "8").c_str());
+  const bool materialize = a.Int("-l", 0) != 0;  // parsed, never consumed by the reference
+  const uint64_t max_matches = static_cast<uint64_t>(a.Int("-L", 1000000));
   std::printf("filename=%s,directed=%d,algo=%d,query_type=%d,execute_mode=%d,compress_level=%d,"
               "enable_ordering=%d,dev_num=%d,budget_gb=%.2f\n",
               filename.c_str(), directed, algo, query, mode, compress, ordering, dev_num, budget_gb);
@@ -132,6 +136,19 @@ int main(int argc, char** argv) {
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   std::printf("batches=%llu\n", static_cast<unsigned long long>(batches));
   std::printf("total_match_count=%llu, elapsed_time=%.3fms\n", total, ms);
+
+  if (materialize) {
+    const int nv = sme_plan_vertices(plan);
+    std::vector<uint32_t> rows(max_matches * static_cast<uint64_t>(nv) + 1);
+    uint64_t listed_total = 0, written = 0;
+    if (sme_list_matches(ctx, g, plan, 0, static_cast<uint32_t>(n), rows.data(), max_matches,
+                         &listed_total, &written) != SME_OK)
+      return Die("match list", ctx);
+    for (uint64_t r = 0; r < written; ++r) {
+      for (int w = 0; w < nv; ++w) std::printf(w ? " %u" : "%u", rows[r * nv + w]);
+      std::printf("\n");
+    }
+  }
 
   sme_graph_destroy(g);
   sme_plan_destroy(plan);

@@ -363,3 +363,23 @@ def test_match_lists(dev, small, q):
     with pytest.raises(sme.SmeError) as e:
         dev.list_matches(dg, sme.Plan(q), max_matches=4)
     assert e.value.status == 5
+
+
+def test_cli_counts_and_match_list(tmp_path, fixture_graphs):
+    """The CLI's result line carries the reference's count, and -l 1 prints
+    the GPSM match list in the oracle's order."""
+    import subprocess
+
+    g = fixture_graphs["rmat8"]
+    p = str(tmp_path / "g.bin")
+    sme.write_bin(p, g)
+    say("case cli -a 4 -q 3 -l 1")
+    r = subprocess.run([sme.CLI_PATH, "-f", p, "-a", "4", "-q", "3", "-m", "1", "-l", "1", "-L", "100000"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.splitlines()
+    i = next(k for k, line in enumerate(lines) if line.startswith("total_match_count="))
+    want, total = oracle.gpsm_list(g.row_ptrs, g.cols, 3)
+    assert int(lines[i].split(",")[0].split("=")[1]) == total
+    got = [[int(x) for x in line.split()] for line in lines[i + 1:]]
+    assert got == want.tolist()
