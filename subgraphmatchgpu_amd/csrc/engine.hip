@@ -240,6 +240,12 @@ struct Frame {
   // candidate segment of the vertex materialized last, per path (for reuse)
   CandRef reuse{};
   bool has_reuse = false;
+  // cand[excl_u] is a view that still holds the values M[excl_v[k]] marked
+  // per path in excl_in (a FILTER_COMPUTE of != conditions left uncopied)
+  int excl_u = -1;
+  uint8_t nexcl = 0;
+  uint8_t excl_v[2]{};
+  const uint8_t* excl_in = nullptr;
 };
 
 ColSet SliceCols(const ColSet& c, uint64_t l) {
@@ -271,6 +277,7 @@ Frame SliceFrame(const Frame& f, uint64_t l, uint64_t r) {
   for (int w = 0; w < kQ; ++w)
     if (f.candmask & (1u << w)) s.cand[w] = SliceCand(f.cand[w], l);
   if (f.has_reuse) s.reuse = SliceCand(f.reuse, l);
+  if (f.excl_in) s.excl_in = f.excl_in + l;
   return s;
 }
 
@@ -641,6 +648,24 @@ class Runner {
     return it;
   }
 
+  static void SetPairExclusion(const Frame& f, int w, uint64_t l, ItemArgs* it) {
+    if (f.excl_u != w || !f.excl_in) return;
+    it->pair_nexcl = f.nexcl;
+    it->pair_excl_v[0] = f.excl_v[0];
+    it->pair_excl_v[1] = f.excl_v[1];
+    it->pair_excl_in = f.excl_in + l;
+  }
+
+  // A FILTER_COMPUTE whose output only feeds the plan's final pair count
+  // (COMPUTE_COUNT with that one unmaterialized vertex) and whose real
+  // conditions beyond the window are at most two != conditions: the count
+  // can subtract the excluded values instead of reading a filtered copy.
+  bool FilterAsView(int L, uint8_t nne) const {
+    if (nne == 0 || nne > 2 || L + 2 != plan_->nops) return false;
+    const OpDesc& nx = plan_->ops[L + 1];
+    return nx.kind == kComputeCount && nx.ncun == 1 && nx.cun[0] == plan_->ops[L].u;
+  }
+
   // --------------------------------------------------------------------------
   void ComputeLevel(int L, const Frame& f) {
     const OpDesc& op = plan_->ops[L];
@@ -717,6 +742,7 @@ class Runner {
       if (pair) {
         pc.it.pair = f.cand[op.cun[0]];
         pc.it.pair_op = op.pair_op;
+        SetPairExclusion(f, op.cun[0], 0, &pc.it);
       }
       pc.it.acc = acc_;
       pc.it.acc_level = acc_ + 1 + L;
@@ -809,6 +835,7 @@ class Runner {
         if (pair) {
           it.pair = SliceCand(f.cand[op.cun[0]], l);
           it.pair_op = op.pair_op;
+          SetPairExclusion(f, op.cun[0], l, &it);
         }
         it.acc = acc_;
         it.acc_level = acc_ + 1 + L;
@@ -1035,6 +1062,32 @@ class Runner {
       Frame f2 = f;
       f2.cand[u] = CandRef{cand.base, pstart.get(), size.get(), nullptr};
       host_level_[L] += W;
+      stats_.batches++;
+      Next(L, f2);
+      return;
+    }
+    if (FilterAsView(L, nne)) {
+      Buf<uint8_t> excl(pool_, P);
+      ExclusionArgs ea{};
+      ea.npaths = P;
+      ea.cs = f.cs;
+      ea.cand = CandRef{cand.base, pstart.get(), size.get(), nullptr};
+      ea.nne = nne;
+      ea.ne_v[0] = ne_v[0];
+      ea.ne_v[1] = nne > 1 ? ne_v[1] : 0;
+      ea.excl_in = excl.get();
+      ea.acc_level = acc_ + 1 + L;
+      {
+        Timed t(ctx_, st_, kSlotFilter, P * (17.0 + 4.0 * nne), P);
+        LaunchExclusion(ea, st_);
+      }
+      Frame f2 = f;
+      f2.cand[u] = ea.cand;
+      f2.excl_u = u;
+      f2.nexcl = nne;
+      f2.excl_v[0] = ea.ne_v[0];
+      f2.excl_v[1] = ea.ne_v[1];
+      f2.excl_in = excl.get();
       stats_.batches++;
       Next(L, f2);
       return;

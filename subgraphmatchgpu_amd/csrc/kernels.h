@@ -129,6 +129,12 @@ struct ItemArgs {
   // COMPUTE_COUNT with one other unmaterialized vertex
   CandRef pair;
   uint8_t pair_op;
+  // values the pair's set excludes without having been copied: bit k of
+  // pair_excl_in[s] says M[pair_excl_v[k]] lies in segment s and is to be
+  // left out (a FILTER_COMPUTE of != conditions kept as a view)
+  uint8_t pair_nexcl;
+  uint8_t pair_excl_v[2];
+  const uint8_t* pair_excl_in;
   // COUNT over two unmaterialized vertices: `pair` is the first vertex's
   // set, `second` the other; swapped[s] == 1 when the second is smaller.
   CandRef second;
@@ -203,6 +209,19 @@ struct CountPathArgs {
   BucketRef bk;
 };
 
+// FILTER_COMPUTE of at most two != conditions kept as a view: per path,
+// bit k of excl_in = [M[ne_v[k]] lies in the path's segment]; the level's
+// output (segment size minus the excluded values present) goes to acc_level.
+struct ExclusionArgs {
+  uint64_t npaths;
+  ColSet cs;
+  CandRef cand;
+  uint8_t nne;
+  uint8_t ne_v[2];
+  uint8_t* excl_in;
+  unsigned long long* acc_level;
+};
+
 // Per-path set-up of the two-set COUNT.
 struct PrepTwoArgs {
   uint64_t npaths;
@@ -218,6 +237,7 @@ void LaunchPrepCompute(const PrepComputeArgs& a, hipStream_t stream);
 void LaunchPrepCount(const PrepCountArgs& a, hipStream_t stream);
 void LaunchPrepList(const PrepListArgs& a, hipStream_t stream);
 void LaunchPrepTwo(const PrepTwoArgs& a, hipStream_t stream);
+void LaunchExclusion(const ExclusionArgs& a, hipStream_t stream);
 void LaunchCountPaths(const CountPathArgs& a, hipStream_t stream);
 void LaunchChunkPartition(const uint64_t* offs, uint64_t nseg, uint64_t nchunks,
                           uint32_t* chunk_seg, hipStream_t stream);

@@ -139,6 +139,25 @@ __device__ __forceinline__ uint64_t CountWithComputed(const uint32_t* arr, uint6
   return n - (eq ? 1 : 0);                     // NON_EQUAL
 }
 
+// CountWithComputed over the pair's set minus the excluded values present
+// (bit k of `in` set: e_k lies in arr).  For !=: n' - [x in arr'];
+// for < / >: the excluded values on the counted side drop out.
+__device__ __forceinline__ uint64_t PairCount(const uint32_t* arr, uint64_t n, uint32_t x, uint8_t op,
+                                              uint8_t nexcl, uint8_t in, uint32_t e0, uint32_t e1) {
+  uint64_t v = CountWithComputed(arr, n, x, op);
+  for (int k = 0; k < nexcl; ++k) {
+    if (!((in >> k) & 1)) continue;
+    const uint32_t e = k == 0 ? e0 : e1;
+    if (op == 0)
+      v -= e > x ? 1 : 0;
+    else if (op == 1)
+      v -= e < x ? 1 : 0;
+    else
+      v -= e == x ? 0 : 1;
+  }
+  return v;
+}
+
 __device__ __forceinline__ uint64_t GridStride() {
   return static_cast<uint64_t>(gridDim.x) * blockDim.x;
 }
@@ -415,7 +434,13 @@ __device__ __forceinline__ u64 GenericChunk(const ItemArgs& a, uint64_t c, int l
         val = ok ? 1 : 0;
       } else if (kMode == kModeSumPair && ok) {
         const uint64_t ps = SegIndex(a.pair, s);
-        val = CountWithComputed(a.pair.base + a.pair.beg[ps], a.pair.len[ps], x, a.pair_op);
+        if (a.pair_nexcl) {
+          val = PairCount(a.pair.base + a.pair.beg[ps], a.pair.len[ps], x, a.pair_op, a.pair_nexcl,
+                          a.pair_excl_in[s], Gather(a.cs, a.pair_excl_v[0], s),
+                          a.pair_nexcl > 1 ? Gather(a.cs, a.pair_excl_v[1], s) : 0u);
+        } else {
+          val = CountWithComputed(a.pair.base + a.pair.beg[ps], a.pair.len[ps], x, a.pair_op);
+        }
       }
     }
   }
@@ -549,10 +574,17 @@ __device__ u64 FastSegment(const ItemArgs& a, uint64_t s, uint64_t c0, uint64_t 
   const uint32_t ne1 = a.nne > 1 ? Gather(a.cs, a.ne_v[1], s) : 0u;
   const uint32_t* parr = nullptr;
   uint64_t plen = 0;
+  uint8_t pin = 0;
+  uint32_t pe0 = 0, pe1 = 0;
   if (kMode == kModeSumPair) {
     const uint64_t ps = SegIndex(a.pair, s);
     parr = a.pair.base + a.pair.beg[ps];
     plen = a.pair.len[ps];
+    if (a.pair_nexcl) {
+      pin = a.pair_excl_in[s];
+      pe0 = Gather(a.cs, a.pair_excl_v[0], s);
+      if (a.pair_nexcl > 1) pe1 = Gather(a.cs, a.pair_excl_v[1], s);
+    }
   }
   u64 local = 0;
   for (uint64_t c = c0; c < c1; ++c) {
@@ -573,7 +605,7 @@ __device__ u64 FastSegment(const ItemArgs& a, uint64_t s, uint64_t c0, uint64_t 
     } else if (kMode == kModeSum) {
       local += ok ? 1 : 0;
     } else if (ok) {
-      local += CountWithComputed(parr, plen, x, a.pair_op);
+      local += PairCount(parr, plen, x, a.pair_op, a.pair_nexcl, pin, pe0, pe1);
     }
   }
   return local;
@@ -655,17 +687,25 @@ __device__ u64 LanePathCount(const PrepCountArgs& a, uint64_t p, uint8_t pv, uin
   const uint32_t* src = (it.has_reuse && pv == it.nb) ? it.reuse.base : it.cols;
   const uint32_t* parr = nullptr;
   uint64_t plen = 0;
+  uint8_t pin = 0;
+  uint32_t pe0 = 0, pe1 = 0;
   if (a.pair) {
     const uint64_t ps = SegIndex(it.pair, p);
     parr = it.pair.base + it.pair.beg[ps];
     plen = it.pair.len[ps];
+    if (it.pair_nexcl) {
+      pin = it.pair_excl_in[p];
+      pe0 = Gather(it.cs, it.pair_excl_v[0], p);
+      if (it.pair_nexcl > 1) pe1 = Gather(it.cs, it.pair_excl_v[1], p);
+    }
   }
   u64 val = 0;
   const int nl = OtherLists(it);
   if (nl > 2 || it.nne > 2) {
     for (uint64_t i = b; i < e; ++i) {
       const uint32_t x = src[i];
-      if (ItemValidPv(it, p, pv, x)) val += a.pair ? CountWithComputed(parr, plen, x, it.pair_op) : 1ull;
+      if (ItemValidPv(it, p, pv, x))
+        val += a.pair ? PairCount(parr, plen, x, it.pair_op, it.pair_nexcl, pin, pe0, pe1) : 1ull;
     }
     return val;
   }
@@ -682,7 +722,7 @@ __device__ u64 LanePathCount(const PrepCountArgs& a, uint64_t p, uint8_t pv, uin
       ok = l1.hrow ? ((l1.hrow[x >> 5] >> (x & 31)) & 1u) != 0 : Gallop(l1.a, &l1.lo, l1.hi, x);
     if (it.nne > 0) ok = ok && x != ne0;
     if (it.nne > 1) ok = ok && x != ne1;
-    if (ok) val += a.pair ? CountWithComputed(parr, plen, x, it.pair_op) : 1ull;
+    if (ok) val += a.pair ? PairCount(parr, plen, x, it.pair_op, it.pair_nexcl, pin, pe0, pe1) : 1ull;
   }
   return val;
 }
@@ -806,6 +846,32 @@ __global__ __launch_bounds__(kThreads) void BucketTableKernel(BucketRef b, uint6
        i += GridStride()) {
     table[i] = static_cast<uint32_t>(BucketOf(b, b.bounds[0] + static_cast<uint32_t>(i)));
   }
+}
+
+__global__ __launch_bounds__(kThreads) void ExclusionKernel(ExclusionArgs a) {
+  u64 local = 0;
+  for (uint64_t p = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; p < a.npaths;
+       p += GridStride()) {
+    const uint64_t seg = SegIndex(a.cand, p);
+    const uint64_t b = a.cand.beg[seg], n = a.cand.len[seg];
+    uint8_t bits = 0;
+    uint64_t drop = 0;
+    for (int k = 0; k < a.nne; ++k) {
+      if (Contains(a.cand.base, b, b + n, Gather(a.cs, a.ne_v[k], p))) {
+        bits |= static_cast<uint8_t>(1u << k);
+        ++drop;
+      }
+    }
+    // two equal excluded values count once
+    if (a.nne == 2 && (bits & 3) == 3 && Gather(a.cs, a.ne_v[0], p) == Gather(a.cs, a.ne_v[1], p)) {
+      bits = 1;
+      drop = 1;
+    }
+    a.excl_in[p] = bits;
+    local += n - drop;
+  }
+  local = WaveSum(local);
+  if ((threadIdx.x & (kWave - 1)) == 0 && local) atomicAdd(a.acc_level, local);
 }
 
 __global__ __launch_bounds__(kThreads) void BuildHubBitsKernel(const uint64_t* rp, const uint32_t* cols,
@@ -978,6 +1044,11 @@ void LaunchPrepCompute(const PrepComputeArgs& a, hipStream_t stream) {
 void LaunchPrepCount(const PrepCountArgs& a, hipStream_t stream) {
   if (a.pa.npaths == 0) return;
   hipLaunchKernelGGL(PrepCountKernel, dim3(GridFor(a.pa.npaths, kThreads)), dim3(kThreads), 0, stream, a);
+}
+
+void LaunchExclusion(const ExclusionArgs& a, hipStream_t stream) {
+  if (a.npaths == 0) return;
+  hipLaunchKernelGGL(ExclusionKernel, dim3(GridFor(a.npaths, kThreads)), dim3(kThreads), 0, stream, a);
 }
 
 void LaunchPrepList(const PrepListArgs& a, hipStream_t stream) {
