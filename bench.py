@@ -86,6 +86,9 @@ def cpu_baseline(g, seconds_target=15.0, chunk=64, seed=12345):
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "intersect_rmat20_traffic.json")
+# Revision of the intersect kernel's memory behaviour; a committed traffic
+# figure is used only when it was measured on the same revision.
+KERNEL_REV = "fast-path-window-stage-1024"
 
 
 def summarize_pmc(fetch_csv, write_csv, calib_bytes, kernel="VerifyKernel"):
@@ -264,7 +267,8 @@ def main():
     try:
         with open(TRAFFIC_FILE) as f:
             tr = json.load(f)
-        if tr.get("config") == {"hub_min_degree": args.hub_min_degree, "schedule": dev.get_tuning()}:
+        if tr.get("config") == {"hub_min_degree": args.hub_min_degree, "schedule": dev.get_tuning(),
+                                "kernel_rev": KERNEL_REV}:
             out["roofline"]["traffic"] = tr["bytes_per_launch"]
             out["roofline"]["traffic_source"] = os.path.relpath(TRAFFIC_FILE, ROOT)
             out["roofline"]["traffic_gbs"] = tr["bytes_per_launch"] / 1e9 / (avg_ms / 1e3)

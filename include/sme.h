@@ -171,7 +171,7 @@ int sme_plan_vertices(const sme_plan* p);
  *                  (0: the level's share of the memory budget).
  *   stream_ratio   pivot rule when a level extends a reused candidate
  *                  segment with one adjacency list: if the segment holds at
- *                  most 512 candidates and the list at most stream_ratio
+ *                  most 1024 candidates and the list at most stream_ratio
  *                  times as many, the list is read in order as the pivot and
  *                  the segment is probed (from LDS) instead of probing the
  *                  list at random (0: the shorter is the pivot).

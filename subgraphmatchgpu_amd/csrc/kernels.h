@@ -67,7 +67,7 @@ struct PrepComputeArgs {
   uint32_t stream_ratio;
 };
 
-constexpr uint32_t kStageEntries = 512;  // LDS entries per wave for a staged list
+constexpr uint32_t kStageEntries = 1024;  // LDS entries per wave for a staged list
 
 // Per-path set-up of a scan over an existing candidate set (MATERIALIZE,
 // FILTER_COMPUTE): the segment is cut to the < / > window.

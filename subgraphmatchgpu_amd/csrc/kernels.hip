@@ -536,10 +536,12 @@ __device__ __forceinline__ bool Member(const ListRef& r, bool staged, const uint
 
 // Chunks [c0, c1) whose items all belong to segment s (at most two other
 // lists and two non-equality conditions).  Everything per segment is loaded
-// once; the first non-hub list of at most kStageCap entries is staged in LDS
-// whole (independent loads: one memory round trip), turning its membership
-// tests into LDS binary searches; a longer list is cut to the value window
-// [x_first, x_last] of the run, the items being one sorted run of the pivot.
+// once.  Every list is cut to the value window [x_first, x_last] of the run
+// (the items are one sorted run of the pivot); the first non-hub list whose
+// window has at most kStageCap entries is staged in LDS (independent loads:
+// one memory round trip), turning its membership tests into LDS binary
+// searches.  A list that is short already is staged whole, without the two
+// searches for its window.
 template <int kMode>
 __device__ u64 FastSegment(const ItemArgs& a, uint64_t s, uint64_t c0, uint64_t c1,
                            uint32_t* stage, int lane) {
@@ -554,10 +556,22 @@ __device__ u64 FastSegment(const ItemArgs& a, uint64_t s, uint64_t c0, uint64_t 
   CollectLists(a, s, pv, &l0, &l1);
   // staged list: 0 = l0, 1 = l1, -1 none
   int staged = -1;
-  if (nl > 0 && !l0.hrow && l0.hi - l0.lo <= kStageCap)
-    staged = 0;
-  else if (nl > 1 && !l1.hrow && l1.hi - l1.lo <= kStageCap)
-    staged = 1;
+  const uint32_t x_first = src[base], x_last = src[base + (t1 - t0) - 1];
+  bool n0 = false, n1 = false;
+  if (nl > 0 && !l0.hrow) {
+    if (l0.hi - l0.lo > kStageCap) {
+      NarrowToWindow(&l0, x_first, x_last);
+      n0 = true;
+    }
+    if (l0.hi - l0.lo <= kStageCap) staged = 0;
+  }
+  if (staged < 0 && nl > 1 && !l1.hrow) {
+    if (l1.hi - l1.lo > kStageCap) {
+      NarrowToWindow(&l1, x_first, x_last);
+      n1 = true;
+    }
+    if (l1.hi - l1.lo <= kStageCap) staged = 1;
+  }
   uint32_t nstage = 0;
   if (staged >= 0) {
     const uint32_t* sa = staged == 0 ? l0.a : l1.a;
@@ -567,9 +581,8 @@ __device__ u64 FastSegment(const ItemArgs& a, uint64_t s, uint64_t c0, uint64_t 
     for (uint32_t i = lane; i < nstage; i += kWave) stage[i] = sa[slo + i];
     WaveSync();
   }
-  const uint32_t x_first = src[base], x_last = src[base + (t1 - t0) - 1];
-  if (nl > 0 && !l0.hrow && staged != 0) NarrowToWindow(&l0, x_first, x_last);
-  if (nl > 1 && !l1.hrow && staged != 1) NarrowToWindow(&l1, x_first, x_last);
+  if (nl > 0 && !l0.hrow && staged != 0 && !n0) NarrowToWindow(&l0, x_first, x_last);
+  if (nl > 1 && !l1.hrow && staged != 1 && !n1) NarrowToWindow(&l1, x_first, x_last);
   const uint32_t ne0 = a.nne > 0 ? Gather(a.cs, a.ne_v[0], s) : 0u;
   const uint32_t ne1 = a.nne > 1 ? Gather(a.cs, a.ne_v[1], s) : 0u;
   const uint32_t* parr = nullptr;
