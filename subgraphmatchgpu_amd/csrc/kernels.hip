@@ -527,20 +527,10 @@ __device__ __forceinline__ void NarrowToWindow(ListRef* r, uint32_t x_first, uin
   r->hi = hi;
 }
 
-// A staged list is either its sorted entries (nstage of them) or, when
-// wbase != kNoBitmap, a bitmap of the ids from 32 * wbase on.
-constexpr uint32_t kNoBitmap = 0xffffffffu;
-// a staged list of at least this many entries whose run covers at most
-// kStageCap words of ids is kept as a bitmap
-constexpr uint32_t kBitmapMinEntries = 32;
-
 __device__ __forceinline__ bool Member(const ListRef& r, bool staged, const uint32_t* stage,
-                                       uint32_t nstage, uint32_t wbase, uint32_t x) {
+                                       uint32_t nstage, uint32_t x) {
   if (r.hrow) return (r.hrow[x >> 5] >> (x & 31)) & 1u;
-  if (staged) {
-    if (wbase != kNoBitmap) return (stage[(x >> 5) - wbase] >> (x & 31)) & 1u;
-    return ContainsLds(stage, nstage, x);
-  }
+  if (staged) return ContainsLds(stage, nstage, x);
   return Contains(r.a, r.lo, r.hi, x);
 }
 
@@ -582,25 +572,13 @@ __device__ u64 FastSegment(const ItemArgs& a, uint64_t s, uint64_t c0, uint64_t 
     }
     if (l1.hi - l1.lo <= kStageCap) staged = 1;
   }
-  uint32_t nstage = 0, wbase = kNoBitmap;
+  uint32_t nstage = 0;
   if (staged >= 0) {
     const uint32_t* sa = staged == 0 ? l0.a : l1.a;
     const uint64_t slo = staged == 0 ? l0.lo : l1.lo;
     nstage = static_cast<uint32_t>((staged == 0 ? l0.hi : l1.hi) - slo);
-    // every item of the run lies in [x_first, x_last]
-    const uint32_t words = (x_last >> 5) - (x_first >> 5) + 1;
     WaveSync();  // earlier readers of the stage are done
-    if (nstage >= kBitmapMinEntries && words <= kStageCap) {
-      wbase = x_first >> 5;
-      for (uint32_t i = lane; i < words; i += kWave) stage[i] = 0u;
-      WaveSync();
-      for (uint32_t i = lane; i < nstage; i += kWave) {
-        const uint32_t v = sa[slo + i];
-        if (v >= x_first && v <= x_last) atomicOr(&stage[(v >> 5) - wbase], 1u << (v & 31));
-      }
-    } else {
-      for (uint32_t i = lane; i < nstage; i += kWave) stage[i] = sa[slo + i];
-    }
+    for (uint32_t i = lane; i < nstage; i += kWave) stage[i] = sa[slo + i];
     WaveSync();
   }
   if (nl > 0 && !l0.hrow && staged != 0 && !n0) NarrowToWindow(&l0, x_first, x_last);
@@ -627,8 +605,8 @@ __device__ u64 FastSegment(const ItemArgs& a, uint64_t s, uint64_t c0, uint64_t 
     bool ok = t < t1;
     uint32_t x = 0;
     if (ok) x = src[base + (t - t0)];
-    if (nl > 0 && ok) ok = Member(l0, staged == 0, stage, nstage, wbase, x);
-    if (nl > 1 && ok) ok = Member(l1, staged == 1, stage, nstage, wbase, x);
+    if (nl > 0 && ok) ok = Member(l0, staged == 0, stage, nstage, x);
+    if (nl > 1 && ok) ok = Member(l1, staged == 1, stage, nstage, x);
     if (a.nne > 0) ok = ok && x != ne0;
     if (a.nne > 1) ok = ok && x != ne1;
     if (kMode == kModeMask) {
