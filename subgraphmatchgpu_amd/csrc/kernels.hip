@@ -534,6 +534,47 @@ __device__ __forceinline__ bool Member(const ListRef& r, bool staged, const uint
   return Contains(r.a, r.lo, r.hi, x);
 }
 
+constexpr int kProbeChunks = 4;
+
+// Membership of x[j] in the sorted a[0, n) for kProbeChunks keys at once:
+// one bisection for all of them (its steps depend on n only), each step
+// issuing the keys' loads together.  The invariant is that the key's lower
+// bound lies in [b, b + len]; at the end it is b or b + 1.
+__device__ __forceinline__ void SearchN(const uint32_t* a, uint32_t n, const uint32_t* x, bool* ok) {
+  uint32_t b[kProbeChunks];
+#pragma unroll
+  for (int j = 0; j < kProbeChunks; ++j) b[j] = 0;
+  uint32_t len = n;
+  while (len > 1) {
+    const uint32_t half = len >> 1;
+#pragma unroll
+    for (int j = 0; j < kProbeChunks; ++j) b[j] = a[b[j] + half - 1] < x[j] ? b[j] + half : b[j];
+    len -= half;
+  }
+#pragma unroll
+  for (int j = 0; j < kProbeChunks; ++j) {
+    const bool at_b = b[j] < n && a[b[j]] == x[j];
+    const bool at_b1 = b[j] + 1 < n && a[b[j] + 1] == x[j];
+    ok[j] = ok[j] && (at_b || at_b1);
+  }
+}
+
+// Member() for kProbeChunks keys; a key whose ok is false is left false.
+__device__ __forceinline__ void MemberN(const ListRef& r, bool staged, const uint32_t* stage,
+                                        uint32_t nstage, const uint32_t* x, bool* ok) {
+  if (r.hrow) {
+    uint32_t w[kProbeChunks];
+#pragma unroll
+    for (int j = 0; j < kProbeChunks; ++j) w[j] = ok[j] ? r.hrow[x[j] >> 5] : 0u;
+#pragma unroll
+    for (int j = 0; j < kProbeChunks; ++j) ok[j] = ok[j] && ((w[j] >> (x[j] & 31)) & 1u);
+  } else if (staged) {
+    SearchN(stage, nstage, x, ok);
+  } else {
+    SearchN(r.a + r.lo, static_cast<uint32_t>(r.hi - r.lo), x, ok);
+  }
+}
+
 // Chunks [c0, c1) whose items all belong to segment s (at most two other
 // lists and two non-equality conditions).  Everything per segment is loaded
 // once.  Every list is cut to the value window [x_first, x_last] of the run
@@ -599,26 +640,36 @@ __device__ u64 FastSegment(const ItemArgs& a, uint64_t s, uint64_t c0, uint64_t 
       if (a.pair_nexcl > 1) pe1 = Gather(a.cs, a.pair_excl_v[1], s);
     }
   }
+  // each lane takes one item of each of kProbeChunks chunks at a time, so
+  // their membership probes are independent loads in flight together
   u64 local = 0;
-  for (uint64_t c = c0; c < c1; ++c) {
-    const uint64_t t = c * kWave + lane;
-    bool ok = t < t1;
-    uint32_t x = 0;
-    if (ok) x = src[base + (t - t0)];
-    if (nl > 0 && ok) ok = Member(l0, staged == 0, stage, nstage, x);
-    if (nl > 1 && ok) ok = Member(l1, staged == 1, stage, nstage, x);
-    if (a.nne > 0) ok = ok && x != ne0;
-    if (a.nne > 1) ok = ok && x != ne1;
-    if (kMode == kModeMask) {
-      const u64 m = __ballot(ok);
-      if (lane == 0) {
-        a.mask[c] = m;
-        a.ccount[c] = static_cast<uint32_t>(__popcll(m));
+  for (uint64_t cb = c0; cb < c1; cb += kProbeChunks) {
+    uint32_t x[kProbeChunks];
+    bool ok[kProbeChunks];
+#pragma unroll
+    for (int j = 0; j < kProbeChunks; ++j) {
+      const uint64_t t = (cb + j) * kWave + lane;
+      ok[j] = cb + j < c1 && t < t1;
+      x[j] = ok[j] ? src[base + (t - t0)] : 0u;
+    }
+    if (nl > 0) MemberN(l0, staged == 0, stage, nstage, x, ok);
+    if (nl > 1) MemberN(l1, staged == 1, stage, nstage, x, ok);
+#pragma unroll
+    for (int j = 0; j < kProbeChunks; ++j) {
+      if (a.nne > 0) ok[j] = ok[j] && x[j] != ne0;
+      if (a.nne > 1) ok[j] = ok[j] && x[j] != ne1;
+      if (cb + j >= c1) continue;  // wave-uniform
+      if (kMode == kModeMask) {
+        const u64 m = __ballot(ok[j]);
+        if (lane == 0) {
+          a.mask[cb + j] = m;
+          a.ccount[cb + j] = static_cast<uint32_t>(__popcll(m));
+        }
+      } else if (kMode == kModeSum) {
+        local += ok[j] ? 1 : 0;
+      } else if (ok[j]) {
+        local += PairCount(parr, plen, x[j], a.pair_op, a.pair_nexcl, pin, pe0, pe1);
       }
-    } else if (kMode == kModeSum) {
-      local += ok ? 1 : 0;
-    } else if (ok) {
-      local += PairCount(parr, plen, x, a.pair_op, a.pair_nexcl, pin, pe0, pe1);
     }
   }
   return local;
