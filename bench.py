@@ -88,7 +88,7 @@ def cpu_baseline(g, seconds_target=15.0, chunk=64, seed=12345):
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "intersect_rmat20_traffic.json")
 # Revision of the intersect kernel's memory behaviour; a committed traffic
 # figure is used only when it was measured on the same revision.
-KERNEL_REV = "fast-path-window-stage-1024"
+KERNEL_REV = "fast-path-window-stage-1024-batched-probes"
 
 
 def summarize_pmc(fetch_csv, write_csv, calib_bytes, kernel="VerifyKernel"):
