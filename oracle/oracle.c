@@ -808,9 +808,31 @@ static void light_run(light_ctx* c, int L) {
   }
 }
 
+static int light_count_impl(uint64_t n, const uint64_t* row_ptrs, const uint32_t* cols, int q,
+                            int compress_level, int enable_ordering, uint32_t v_begin,
+                            uint32_t v_end, int nthreads, uint64_t* out_count, uint64_t* level_out,
+                            uint64_t* per_vertex);
+
 int or_light_count(uint64_t n, const uint64_t* row_ptrs, const uint32_t* cols, int q,
                    int compress_level, int enable_ordering, uint32_t v_begin, uint32_t v_end,
                    int nthreads, uint64_t* out_count, uint64_t* level_out) {
+  return light_count_impl(n, row_ptrs, cols, q, compress_level, enable_ordering, v_begin, v_end,
+                          nthreads, out_count, level_out, NULL);
+}
+
+int or_light_count_per_vertex(uint64_t n, const uint64_t* row_ptrs, const uint32_t* cols, int q,
+                              int compress_level, int enable_ordering, uint32_t v_begin,
+                              uint32_t v_end, int nthreads, uint64_t* per_vertex) {
+  uint64_t total;
+  if (v_end > n) return -2;
+  return light_count_impl(n, row_ptrs, cols, q, compress_level, enable_ordering, v_begin, v_end,
+                          nthreads, &total, NULL, per_vertex);
+}
+
+static int light_count_impl(uint64_t n, const uint64_t* row_ptrs, const uint32_t* cols, int q,
+                            int compress_level, int enable_ordering, uint32_t v_begin,
+                            uint32_t v_end, int nthreads, uint64_t* out_count, uint64_t* level_out,
+                            uint64_t* per_vertex) {
   or_plan p;
   if (or_light_plan(q, compress_level, enable_ordering, &p)) return -1;
   if (v_end > n) v_end = (uint32_t)n;
@@ -829,9 +851,11 @@ int or_light_count(uint64_t n, const uint64_t* row_ptrs, const uint32_t* cols, i
     c.tmp = (uint32_t*)malloc(md * sizeof(uint32_t));
 #pragma omp for schedule(dynamic, 64)
     for (int64_t v = v_begin; v < (int64_t)v_end; ++v) {
+      const uint64_t before = c.count;
       c.M[p.seq[0]] = (uint32_t)v; /* InitFirstLevel, LIGHTMaterialize.cuh:12-21 */
       c.level[0]++;
       light_run(&c, 1);
+      if (per_vertex) per_vertex[v - v_begin] = c.count - before;
     }
 #pragma omp critical
     {

@@ -93,6 +93,12 @@ int or_light_count(uint64_t n, const uint64_t* row_ptrs, const uint32_t* cols,
                    uint32_t v_begin, uint32_t v_end, int nthreads,
                    uint64_t* out_count, uint64_t* level_out);
 
+/* or_light_count split by start vertex: per_vertex[v - v_begin] receives the
+ * count of the embeddings whose first matched vertex is v (v_end <= n). */
+int or_light_count_per_vertex(uint64_t n, const uint64_t* row_ptrs, const uint32_t* cols, int q,
+                              int compress_level, int enable_ordering, uint32_t v_begin,
+                              uint32_t v_end, int nthreads, uint64_t* per_vertex);
+
 /* GPSM semantics (gpu/impl/gpsm/HybridGPSMGPUProcessor.cuh:23-205): per
  * level, intersect the ordered connectivity with whole-ordered conditions and
  * materialize; level_out[l] = partial instances after level l (l < nv). */
@@ -103,6 +109,22 @@ int or_gpsm_count(uint64_t n, const uint64_t* row_ptrs, const uint32_t* cols,
 int or_gpsm_list(uint64_t n, const uint64_t* row_ptrs, const uint32_t* cols, int q,
                  int enable_ordering, uint32_t v_begin, uint32_t v_end, uint32_t* out,
                  uint64_t cap, uint64_t* rows, uint64_t* total);
+
+/* Whole-graph pattern counts by degree-ordered clique listing (census.c):
+ * an independent check of the LIGHT counts on large graphs, sharing no code
+ * or ordering with the functions above.  kmax (3..5) is the largest clique
+ * listed; triangles and diamonds are always counted. */
+typedef struct {
+  uint64_t edges;           /* undirected edges (self loops excluded) */
+  uint64_t self_loops;
+  uint64_t triangles;       /* Q0 */
+  uint64_t diamonds;        /* Q2: sum over edges of C(t(e), 2) */
+  uint64_t k4;              /* Q3 (kmax >= 4) */
+  uint64_t k5;              /* Q7 (kmax >= 5) */
+  uint64_t max_out_degree;  /* of the degree orientation */
+} or_census_t;
+int or_census(uint64_t n, const uint64_t* row_ptrs, const uint32_t* cols, int kmax, int nthreads,
+              or_census_t* out);
 
 #ifdef __cplusplus
 }

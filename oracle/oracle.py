@@ -110,6 +110,25 @@ def light_count(rp, cols, q, compress_level=3, enable_ordering=True, v_begin=0, 
     return int(out.value), [int(lv[i]) for i in range(nops)]
 
 
+def light_count_per_vertex(rp, cols, q, v_begin=0, v_end=None, compress_level=3, enable_ordering=True,
+                           threads=0):
+    """LIGHT count of every start vertex in [v_begin, v_end) (uint64 array)."""
+    rp, cols = _arrays(rp, cols)
+    n = rp.size - 1
+    v_end = n if v_end is None else v_end
+    out = np.zeros(max(0, v_end - v_begin), dtype=np.uint64)
+    L = lib()
+    L.or_light_count_per_vertex.argtypes = [ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                            ctypes.c_int, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32,
+                                            ctypes.c_int, ctypes.c_void_p]
+    if out.size:
+        rc = L.or_light_count_per_vertex(n, rp.ctypes.data, cols.ctypes.data, q, compress_level,
+                                         int(enable_ordering), v_begin, v_end, threads, out.ctypes.data)
+        if rc:
+            raise ValueError(rc)
+    return out
+
+
 def gpsm_count(rp, cols, q, enable_ordering=True, v_begin=0, v_end=None, threads=0):
     rp, cols = _arrays(rp, cols)
     n = rp.size - 1
@@ -120,6 +139,26 @@ def gpsm_count(rp, cols, q, enable_ordering=True, v_begin=0, v_end=None, threads
     if rc:
         raise ValueError(rc)
     return int(out.value), [int(lv[i]) for i in range(MAX_OPS)]
+
+
+class _Census(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_uint64) for k in ("edges", "self_loops", "triangles", "diamonds", "k4", "k5",
+                                               "max_out_degree")]
+
+
+def census(rp, cols, kmax=4, threads=0) -> dict:
+    """Whole-graph counts by degree-ordered clique listing (census.c), an
+    independent check of the plan-following functions: edges, triangles
+    (Q0), diamonds (Q2), 4-cliques (Q3, kmax >= 4), 5-cliques (Q7, kmax 5)."""
+    rp, cols = _arrays(rp, cols)
+    L = lib()
+    L.or_census.argtypes = [ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                            ctypes.POINTER(_Census)]
+    out = _Census()
+    rc = L.or_census(rp.size - 1, rp.ctypes.data, cols.ctypes.data, kmax, threads, ctypes.byref(out))
+    if rc:
+        raise ValueError(rc)
+    return {k: int(getattr(out, k)) for k, _ in _Census._fields_}
 
 
 def ref_available() -> bool:

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <chrono>
 #include <memory>
+#include <new>
 #include <cstring>
 #include <utility>
 
@@ -1227,6 +1228,19 @@ class Runner {
   RunStats stats_;
 };
 
+// After a failed run: drain the stream so that pool memory handed back is no
+// longer in use by queued kernels, and recycle the run's timing events.
+Status Recover(Context* ctx, Status s, const std::string& msg, std::string* err) {
+  (void)hipStreamSynchronize(ctx->stream());
+  for (auto& m : ctx->marks) {
+    ctx->spare_events.push_back(m.start);
+    ctx->spare_events.push_back(m.stop);
+  }
+  ctx->marks.clear();
+  *err = msg;
+  return s;
+}
+
 }  // namespace
 
 Status RunPlan(Context* ctx, const DevGraph* g, const EnginePlan* plan, uint32_t v_begin,
@@ -1255,15 +1269,9 @@ Status RunPlan(Context* ctx, const DevGraph* g, const EnginePlan* plan, uint32_t
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (stats) *stats = st;
   } catch (const Error& e) {
-    // drain the stream so pool memory handed back is no longer in use
-    (void)hipStreamSynchronize(ctx->stream());
-    for (auto& m : ctx->marks) {
-      ctx->spare_events.push_back(m.start);
-      ctx->spare_events.push_back(m.stop);
-    }
-    ctx->marks.clear();
-    *err = e.msg;
-    return e.status;
+    return Recover(ctx, e.status, e.msg, err);
+  } catch (const std::bad_alloc&) {
+    return Recover(ctx, kErrOom, "host memory exhausted", err);
   }
   return kOk;
 }
@@ -1294,14 +1302,9 @@ Status ListMatches(Context* ctx, const DevGraph* g, const EnginePlan* plan, uint
     if (ctx->profiling) ctx->ResolveMarks();
     *rows = r.ListedRows();
   } catch (const Error& e) {
-    (void)hipStreamSynchronize(ctx->stream());
-    for (auto& m : ctx->marks) {
-      ctx->spare_events.push_back(m.start);
-      ctx->spare_events.push_back(m.stop);
-    }
-    ctx->marks.clear();
-    *err = e.msg;
-    return e.status;
+    return Recover(ctx, e.status, e.msg, err);
+  } catch (const std::bad_alloc&) {
+    return Recover(ctx, kErrOom, "host memory exhausted", err);
   }
   return kOk;
 }
@@ -1355,14 +1358,9 @@ Status RunPlanBuckets(Context* ctx, const DevGraph* g, const EnginePlan* plan, c
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (stats) *stats = st;
   } catch (const Error& e) {
-    (void)hipStreamSynchronize(ctx->stream());
-    for (auto& m : ctx->marks) {
-      ctx->spare_events.push_back(m.start);
-      ctx->spare_events.push_back(m.stop);
-    }
-    ctx->marks.clear();
-    *err = e.msg;
-    return e.status;
+    return Recover(ctx, e.status, e.msg, err);
+  } catch (const std::bad_alloc&) {
+    return Recover(ctx, kErrOom, "host memory exhausted", err);
   }
   return kOk;
 }

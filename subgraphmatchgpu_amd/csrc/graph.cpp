@@ -57,6 +57,25 @@ Status ReadBin(const std::string& path, HostGraph* g, std::string* err) {
     *err = "BIN id widths are not (4, 8) in " + path;
     return kErrInvalid;
   }
+  // the header's sizes must match the file before anything is allocated
+  if (std::fseek(f, 0, SEEK_END) != 0) {
+    std::fclose(f);
+    *err = "cannot seek in " + path;
+    return kErrIo;
+  }
+  const long fsize = std::ftell(f);
+  if (fsize < 0 || std::fseek(f, 4 * sizeof(uint64_t), SEEK_SET) != 0) {
+    std::fclose(f);
+    *err = "cannot seek in " + path;
+    return kErrIo;
+  }
+  const uint64_t body = static_cast<uint64_t>(fsize) - 4 * sizeof(uint64_t);
+  if (hdr[2] >= 0xffffffffull || hdr[3] > body / sizeof(uintV) ||
+      (hdr[2] + 1) * sizeof(uintE) + hdr[3] * sizeof(uintV) != body) {
+    std::fclose(f);
+    *err = "BIN header sizes do not match the file size of " + path;
+    return kErrInvalid;
+  }
   g->n = hdr[2];
   g->nnz = hdr[3];
   g->row_ptrs.resize(g->n + 1);

@@ -210,15 +210,19 @@ void Server::Dispatch() {
 }
 
 Status Server::RunJob(int slot, const Job& job, const std::vector<Request>& snap,
-                      std::vector<uint64_t>* counts, std::string* err) {
+                      std::vector<uint64_t>* counts, std::string* err, uint64_t* peak) {
   Context* c = ctx_[slot].get();
   c->limit = job.reserve;
   const EnginePlan* plan = &plans_[job.q];
   counts->assign(snap.size(), 0);
+  *peak = 0;
   if (snap.size() == 1) {
-    return RunPlan(c, graph_.get(), plan, snap[0].vb, snap[0].ve, &(*counts)[0], nullptr, nullptr,
-                   err);
+    const Status s = RunPlan(c, graph_.get(), plan, snap[0].vb, snap[0].ve, &(*counts)[0], nullptr,
+                             nullptr, err);
+    *peak = c->held_peak;
+    return s;
   }
+  uint64_t& job_peak = *peak;
   // maximal intervals of the union; one bucketed pass per interval with the
   // requests' end points as bucket bounds
   std::vector<size_t> order(snap.size());
@@ -240,6 +244,9 @@ Status Server::RunJob(int slot, const Job& job, const std::vector<Request>& snap
     uint64_t total = 0;
     const Status s = RunPlanBuckets(c, graph_.get(), plan, bounds.data(), bucket.size(), bucket.data(),
                                     &total, nullptr, nullptr, err);
+    // each interval's run restarts the context's high-water mark; the job's
+    // peak is the largest of them
+    job_peak = std::max(job_peak, c->held_peak);
     if (s != kOk) return s;
     // prefix over buckets: count of [bounds[a], bounds[b]) = pre[b] - pre[a]
     std::vector<uint64_t> pre(bucket.size() + 1, 0);
@@ -268,13 +275,19 @@ void Server::Work(int slot) {
     std::vector<uint64_t> counts;
     std::string err;
     Status s = kOk;
+    uint64_t peak = 0;
     try {
-      s = RunJob(slot, *job, snap, &counts, &err);
+      s = RunJob(slot, *job, snap, &counts, &err, &peak);
     } catch (const Error& e) {
       s = e.status;
       err = e.msg;
+    } catch (const std::bad_alloc&) {
+      s = kErrOom;
+      err = "host memory exhausted";
+    } catch (const std::exception& e) {
+      s = kErrInvalid;
+      err = e.what();
     }
-    const uint64_t peak = ctx_[slot]->held_peak;
     lk.lock();
     const Clock::time_point now = Clock::now();
     for (size_t k = 0; k < job->ids.size(); ++k) {

@@ -84,9 +84,10 @@ class Server {
   void Dispatch();
   void Work(int slot);
   uint64_t WorkUnits(uint32_t vb, uint32_t ve) const;
-  // Runs one job on `slot`'s context; fills counts per request id.
+  // Runs one job on `slot`'s context; fills counts per request id and *peak
+  // with the most pool bytes the job held at once (over all its passes).
   Status RunJob(int slot, const Job& job, const std::vector<Request>& snap,
-                std::vector<uint64_t>* counts, std::string* err);
+                std::vector<uint64_t>* counts, std::string* err, uint64_t* peak);
 
   ServerConfig cfg_;
   DevicePool pool_;

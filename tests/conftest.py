@@ -33,6 +33,12 @@ def make_fixture_graph(spec):
 
 
 @pytest.fixture(scope="session")
+def config_counts():
+    with open(os.path.join(GOLDEN, "config_counts.json")) as f:
+        return json.load(f)
+
+
+@pytest.fixture(scope="session")
 def fixture_graphs(ref_counts):
     return {name: make_fixture_graph(spec) for name, spec in ref_counts["graphs"].items()}
 

@@ -287,11 +287,25 @@ def test_autotune_installs_a_measured_setting(fixture_graphs):
     d.close()
 
 
+_PER_VERTEX = {}
+
+
+def oracle_prefix(name, g, q, lo=0, hi=None):
+    """Prefix sums of the oracle's per-start-vertex LIGHT counts over
+    [lo, hi): the count of a range [a, b) inside it is P[b - lo] - P[a - lo]."""
+    hi = g.n if hi is None else hi
+    key = (name, q, lo, hi)
+    if key not in _PER_VERTEX:
+        pv = oracle.light_count_per_vertex(g.row_ptrs, g.cols, q, lo, hi)
+        _PER_VERTEX[key] = np.concatenate([[0], np.cumsum(pv, dtype=np.uint64)]).astype(np.uint64)
+    return _PER_VERTEX[key]
+
+
 @pytest.mark.parametrize("slots,merge,hub", [(4, True, 0), (4, True, 64), (1, False, 0), (2, True, 0)])
 def test_server_results_alone_and_in_the_mix(fixture_graphs, slots, merge, hub):
     """Each request's count from the server (merged bucketed passes running
     concurrently on several streams of one shared pool) equals its count
-    alone through sme_count."""
+    alone through sme_count and the oracle's count of its range."""
     g = fixture_graphs["rmat12"]
     rng = np.random.default_rng(100 + slots)
     reqs = []
@@ -312,21 +326,108 @@ def test_server_results_alone_and_in_the_mix(fixture_graphs, slots, merge, hub):
     st = srv.stats()
     srv.close()
     assert got == alone
+    want = [int(oracle_prefix("rmat12", g, q)[b] - oracle_prefix("rmat12", g, q)[a]) for q, a, b in reqs]
+    assert got == want
     assert st["requests_done"] == len(reqs) and st["failed"] == 0
     if merge:
         assert st["passes"] < len(reqs)
     assert st["peak_reserved"] <= 2 << 30
 
 
-def test_config3_rmat22_slices():
+@pytest.fixture(scope="module")
+def rmat22():
+    """R-MAT scale 22, edge factor 32 (the memory-capped and the served
+    configurations' graph), generated once per module."""
+    say("generating rmat22 ef32")
+    g = sme.gen_rmat(22, 32, seed=1)
+    assert g.n == 1 << 22
+    return g
+
+
+# Start-vertex windows of R-MAT 22 ef32 where every pattern has instances and
+# the oracle counts a window in about a second (R-MAT degrees fall with the
+# number of set bits of the id, not with the id).
+R22_WINDOWS = {q: [(2500000, 2501024), (2900000, 2900512)] for q in (0, 1, 2, 3, 7)}
+R22_WINDOWS[4] = [(2500000, 2500512)]
+
+
+def test_server_rmat22_requests_vs_oracle(rmat22):
+    """The served configuration's graph stays resident in the server; a
+    seeded request mix with the served pattern weights, each request a random
+    start range inside a window the oracle finishes in seconds, runs through
+    4 concurrent slots with merging.  Every count equals the oracle's count
+    of its range and the count of the same request alone through sme_count."""
+    g = rmat22
+    rng = np.random.default_rng(22)
+    reqs = []
+    for _ in range(60):
+        q = int(rng.choice([0, 1, 2, 3, 4, 7], p=[0.3, 0.2, 0.2, 0.15, 0.1, 0.05]))
+        wins = R22_WINDOWS[q]
+        lo, hi = wins[int(rng.integers(0, len(wins)))]
+        ln = max(1, int(np.ceil(rng.uniform(0.01, 1.0) * (hi - lo))))
+        a = int(rng.integers(lo, hi - ln + 1))
+        reqs.append((q, a, a + ln))
+    say("case server rmat22: oracle counts of the windows")
+    want = []
+    for q, a, b in reqs:
+        lo, hi = next(w for w in R22_WINDOWS[q] if w[0] <= a and b <= w[1])
+        P = oracle_prefix("rmat22", g, q, lo, hi)
+        want.append(int(P[b - lo] - P[a - lo]))
+    assert sum(1 for w in want if w) >= 30
+    say("case server rmat22: served")
+    srv = sme.Server(g, mem_cap_bytes=64 << 30, slots=4, merge=True, min_reserve_bytes=1 << 30,
+                     hub_min_degree=256, hub_max_bytes=4 << 30)
+    ids = [srv.submit(q, a, b) for q, a, b in reqs]
+    got = [srv.wait(i)[0] for i in ids]
+    st = srv.stats()
+    srv.close()
+    assert got == want
+    assert st["requests_done"] == len(reqs) and st["failed"] == 0 and st["passes"] < len(reqs)
+    say("case server rmat22: alone")
+    d = sme.Device(0, budget_bytes=16 << 30)
+    dg = d.upload(g)
+    alone = [d.count(dg, sme.Plan(q), a, b).count for q, a, b in reqs]
+    d.close()
+    assert alone == want
+
+
+def test_config2_rmat20_full(config_counts):
+    """Configuration 2 at full size: R-MAT scale 20, edge factor 16, every
+    start vertex, 4-clique and diamond.  Counts and per-level outputs equal
+    the committed fixture (independent census, and the reference binary's
+    count where it finished), with and without hub bitmaps; GPSM_REUSE agrees
+    with LIGHT on a start range."""
+    ent = config_counts["rmat20"]
+    g = sme.gen_rmat(20, 16, seed=1)
+    assert g.checksum() == ent["sha256"]
+    for q, ref in ent.get("reference", {}).items():
+        assert ref["count"] == ent["light"][q]["count"]
+    for hub in (256, 0):
+        d = sme.Device(0, budget_bytes=32 << 30)
+        if hub:
+            d.set_hub_bitmaps(hub, 8 << 30)
+        dg = d.upload(g)
+        for q in (sme.Q3, sme.Q2):
+            say(f"case config2 q={q} hub={hub}")
+            r = d.count(dg, sme.Plan(q))
+            want = ent["light"][str(q)]
+            assert (r.count, r.levels) == (want["count"], want["levels"])
+        if hub:
+            a, b = 1 << 16, 1 << 20
+            say(f"case config2 gpsm_reuse q=3 range=[{a},{b})")
+            light = d.count(dg, sme.Plan(sme.Q3), a, b).count
+            assert d.count(dg, sme.Plan(sme.Q3, sme.GPU_GPSM_REUSE), a, b).count == light
+        d.close()
+
+
+def test_config3_rmat22_slices(rmat22):
     """RMAT scale 22, edge factor 32 (the memory-capped configuration), full
     CSR resident, 5-clique and house on start-vertex slices: counts and
     per-level outputs equal the LIGHT oracle where it finishes in seconds;
     on a heavier slice a 32 GB and a 64 MB budget give identical counts and
     levels (the smaller in more batches) and GPSM_REUSE agrees on the
     count."""
-    g = sme.gen_rmat(22, 32, seed=1)
-    assert g.n == 1 << 22
+    g = rmat22
     big = sme.Device(0, budget_bytes=32 << 30)
     small = sme.Device(0, budget_bytes=64 << 20)
     gb, gs = big.upload(g), small.upload(g)

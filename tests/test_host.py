@@ -64,6 +64,13 @@ def test_bin_errors(tmp_path):
         sme.read_bin(str(p))
     with pytest.raises(sme.SmeError):
         sme.read_bin(str(tmp_path / "missing.bin"))
+    # header sizes that the file cannot hold are rejected before any
+    # allocation (n = 2^64 - 1 would wrap n + 1 to 0)
+    for n, nnz in ((2**64 - 1, 0), (2**40, 2**40), (3, 2**62), (1, 3)):
+        p.write_bytes(np.array([4, 8, n, nnz], dtype=np.uint64).tobytes() + bytes(8 * 2))
+        with pytest.raises(sme.SmeError) as e:
+            sme.read_bin(str(p))
+        assert e.value.status == 1
 
 
 def test_snap_reader(tmp_path):

@@ -137,6 +137,81 @@ def test_triangle_levels_closed_form(fixture_graphs):
         assert lv == [g.n, g.nnz // 2, g.nnz // 2, cnt]
 
 
+def kn_level_forms(q, n):
+    """Per-exec-level outputs of the default (SPECIAL) LIGHT plan on K_n,
+    derived by hand from each op's semantics:
+      Q1 C2 = sum over v0 < v1 of |{v2 > v0, v2 != v1}| = 2 C(n,3);
+      Q2 CPC1 = CPC3 = sum over v0 < v2 of |N(v0) & N(v2)| = C(n,2)(n-2);
+      Q4 C3 = all of N(v1) (n(n-1)), C0 = M3 = C(n,2)(n-2) = 3 C(n,3),
+         FC0 drops v3 from cand0: 3 C(n,3)(n-3) = 12 C(n,4)."""
+    C = math.comb
+    c2, c3, c4, c5 = C(n, 2), C(n, 3), C(n, 4), C(n, 5)
+    return {0: [n, c2, c2, c3],
+            1: [n, c2, c2, 2 * c3, 2 * c3, 3 * c4],
+            2: [n, c2, c2, 3 * c3, 3 * c3, 6 * c4],
+            3: [n, c2, c2, c3, c3, c4],
+            4: [n, c2, n * (n - 1), c2, 3 * c3, 3 * c3, 12 * c4, 60 * c5],
+            7: [n, c2, c2, c3, c3, c4, c4, c5]}[q]
+
+
+@pytest.mark.parametrize("n", [5, 6, 8, 11])
+@pytest.mark.parametrize("q", [0, 1, 2, 3, 4, 7])
+def test_light_levels_closed_form_cliques(q, n):
+    g = clique(n)
+    count, levels = oracle.light_count(g.row_ptrs, g.cols, q, 3)
+    assert levels == kn_level_forms(q, n)
+    assert count == kn_closed_form(q, n)
+
+
+def test_census_matches_reference_binary(ref_counts, fixture_graphs):
+    """The independent census (degree-ordered clique listing, per-edge
+    triangle support) reproduces the reference binary's triangle, diamond,
+    4-clique and 5-clique counts on every fixture graph."""
+    for name, res in ref_counts["counts"].items():
+        g = fixture_graphs[name]
+        c = oracle.census(g.row_ptrs, g.cols, kmax=5)
+        assert c["edges"] == g.nnz // 2
+        for q, key in (("0", "triangles"), ("2", "diamonds"), ("3", "k4"), ("7", "k5")):
+            if q in res["wcoj"]:
+                assert c[key] == res["wcoj"][q], (name, q)
+
+
+def test_census_cliques_and_brute_force():
+    import subgraphmatchgpu_amd as sme
+
+    for n in (4, 7, 10):
+        c = oracle.census(clique(n).row_ptrs, clique(n).cols, kmax=5)
+        assert (c["triangles"], c["diamonds"], c["k4"], c["k5"]) == (
+            kn_closed_form(0, n), kn_closed_form(2, n), kn_closed_form(3, n), kn_closed_form(7, n))
+    rng = random.Random(7)
+    n = 9
+    adj = [set() for _ in range(n)]
+    for a, b in itertools.combinations(range(n), 2):
+        if rng.random() < 0.6:
+            adj[a].add(b)
+            adj[b].add(a)
+    lists = [sorted(s) for s in adj]
+    g = sme.from_adjacency(lists)
+    c = oracle.census(g.row_ptrs, g.cols, kmax=5)
+    for q, key in ((0, "triangles"), (2, "diamonds"), (3, "k4"), (7, "k5")):
+        assert c[key] == brute_monomorphisms(lists, q) // AUT[q], q
+
+
+def test_light_levels_from_census(fixture_graphs):
+    """On R-MAT graphs the clique and diamond plans' level outputs are whole
+    graph counts: Q3 [n, m, m, T, T, K4], Q7 [.., K4, K4, K5], Q2 [n, m, m,
+    3T, 3T, D] (each triangle has three chord candidates)."""
+    for name in ("rmat8", "rmat12"):
+        g = fixture_graphs[name]
+        c = oracle.census(g.row_ptrs, g.cols, kmax=5)
+        m, T = c["edges"], c["triangles"]
+        assert oracle.light_count(g.row_ptrs, g.cols, 3, 3) == (c["k4"], [g.n, m, m, T, T, c["k4"]])
+        assert oracle.light_count(g.row_ptrs, g.cols, 7, 3) == (
+            c["k5"], [g.n, m, m, T, T, c["k4"], c["k4"], c["k5"]])
+        assert oracle.light_count(g.row_ptrs, g.cols, 2, 3) == (c["diamonds"], [g.n, m, m, 3 * T, 3 * T,
+                                                                               c["diamonds"]])
+
+
 def test_level_outputs_consistent(fixture_graphs):
     """Every compress level gives the same total; GPSM's last level is the
     count and its first level is the start range."""
@@ -155,6 +230,17 @@ def test_start_range_additivity(fixture_graphs):
         parts = sum(oracle.light_count(g.row_ptrs, g.cols, q, 3, v_begin=a, v_end=b)[0]
                     for a, b in ((0, 100), (100, 517), (517, g.n)))
         assert parts == whole
+
+
+def test_per_vertex_counts_add_up(fixture_graphs):
+    """Per-start-vertex counts sum to the range counts and equal one-vertex
+    ranges."""
+    g = fixture_graphs["rmat10"]
+    for q in (0, 1, 2, 3, 4, 7):
+        pv = oracle.light_count_per_vertex(g.row_ptrs, g.cols, q, 100, 900)
+        assert int(pv.sum()) == oracle.light_count(g.row_ptrs, g.cols, q, 3, v_begin=100, v_end=900)[0]
+        for v in (100, 101, 350, 899):
+            assert int(pv[v - 100]) == oracle.light_count(g.row_ptrs, g.cols, q, 3, v_begin=v, v_end=v + 1)[0]
 
 
 @pytest.mark.skipif(not oracle.ref_available(), reason="reference CPU binary not built")
