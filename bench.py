@@ -88,7 +88,7 @@ def cpu_baseline(g, seconds_target=15.0, chunk=64, seed=12345):
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "intersect_rmat20_traffic.json")
 # Revision of the intersect kernel's memory behaviour; a committed traffic
 # figure is used only when it was measured on the same revision.
-KERNEL_REV = "fast-path-window-stage-1024-batched-probes"
+KERNEL_REV = "ranked-copy-group-count-bucket-hash-tiles64"
 
 
 def summarize_pmc(fetch_csv, write_csv, calib_bytes, kernel="VerifyKernel"):
@@ -144,8 +144,11 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--autotune", action="store_true",
                     help="rmat20: let the autotuner pick the schedule before timing")
-    ap.add_argument("--tune", default="0,0,0,4",
-                    help="lane_path_max,item_grid,batch_bytes,super_chunks[,stream_ratio] schedule (rmat20)")
+    ap.add_argument("--tune", default="0,0,0,4,0,512",
+                    help="lane_path_max,item_grid,batch_bytes,super_chunks[,stream_ratio[,group_max]] "
+                         "schedule (rmat20)")
+    ap.add_argument("--no-rank-order", action="store_true",
+                    help="rmat20: no degree-ranked copy (clique counts then walk the original ids)")
     ap.add_argument("--summarize-pmc", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "CALIB_MIB"),
                     help="print HBM traffic per intersect launch from two rocprofv3 --pmc runs")
     args = ap.parse_args()
@@ -176,14 +179,18 @@ def main():
             f"({2 * (args.calib_copy_mib << 20) / ms / 1e6:.1f} GB/s read+write)")
     if args.hub_min_degree:
         dev.set_hub_bitmaps(args.hub_min_degree, int(args.hub_max_gb * (1 << 30)))
+    dev.set_rank_order(not args.no_rank_order)
+    # reference-like set-up (PatternMatch.cuh:58-87 times plan + CSR upload):
+    # plans, CSR upload, hub bitmaps and the degree-ranked copy
     t0 = time.perf_counter()
-    dg = dev.upload(g)
-    log(f"upload: {1e3 * (time.perf_counter() - t0):.1f} ms, {dg.hubs} hub bitmaps")
     plans = [sme.Plan(q, sme.GPU_LIGHT, sme.CL_SPECIAL) for q in QUERIES]
+    dg = dev.upload(g)
+    setup_s = time.perf_counter() - t0
+    log(f"plans + upload: {1e3 * setup_s:.1f} ms, {dg.hubs} hub bitmaps, ranked copy {dg.ranked}")
     if args.tune:
-        f = [int(x) for x in args.tune.split(",")] + [0, 0, 0, 0, 0]
+        f = [int(x) for x in args.tune.split(",")] + [0, 0, 0, 0, 0, 0]
         dev.set_tuning(lane_path_max=f[0], item_grid=f[1], batch_bytes=f[2], super_chunks=f[3],
-                       stream_ratio=f[4])
+                       stream_ratio=f[4], group_max=f[5])
     if args.autotune:
         t = time.perf_counter()
         best, ms, trials = dev.autotune(dg, plans, sample_fraction=0.02)
@@ -198,7 +205,7 @@ def main():
             counts.append(r.count)
             log(f"  Q{q}: {r.count} embeddings in {1e3 * (time.perf_counter() - t):.1f} ms, "
                 f"{r.batches} batches, {r.d2h_syncs} syncs, peak pool {r.peak_pool_bytes / 2**30:.2f} GiB, "
-                f"levels {r.levels}")
+                f"ranked copy {r.ranked}, levels {r.levels}")
         return counts
 
     for i in range(args.warmup):
@@ -220,7 +227,19 @@ def main():
     ms_per_step = 1000.0 * elapsed / args.steps
     value = sum(counts) * args.steps / elapsed
 
-    dom = prof["intersect"]
+    # the dominant intersection family: item verify kernels, group counting
+    # or lane counting, whichever took the most time
+    fam = max(("intersect", "intersect_group", "intersect_lane"), key=lambda k: prof[k]["ms"])
+    dom = prof[fam]
+
+    def family_roofline(names):
+        ms = sum(prof[k]["ms"] for k in names)
+        launches = sum(prof[k]["launches"] for k in names)
+        by = sum(prof[k]["bytes"] for k in names)
+        gbs = (by / 1e9) / (ms / 1e3) if ms > 0 else 0.0
+        return {"kernels": list(names), "launches": launches, "ms_per_step": ms / args.steps,
+                "GB_per_step": by / 1e9 / args.steps, "achieved": gbs, "frac": gbs / HBM_PEAK_GBS}
+
     avg_ms = dom["ms"] / max(1, dom["launches"])
     bytes_per_launch = dom["bytes"] / max(1, dom["launches"])
     achieved = (bytes_per_launch / 1e9) / (avg_ms / 1e3) if avg_ms > 0 else 0.0
@@ -250,9 +269,18 @@ def main():
                                     "GB": round(v["bytes"] / 1e9, 3), "items": v["items"]}
                                 for k, v in prof.items()},
         },
+        "reference_like": {
+            "value": sum(counts) / (setup_s + elapsed / args.steps),
+            "unit": "embeddings/s",
+            "definition": "one step's embeddings / (plan creation + CSR upload with hub bitmaps and the "
+                          "degree-ranked copy + one step), as PatternMatch.cuh:58-87 times plan and upload",
+            "setup_ms": 1e3 * setup_s,
+        },
         "roofline": {
             "bound": "hbm",
-            "kernel": "intersect (VerifyKernel)",
+            "kernel": {"intersect": "intersect (VerifyKernel)",
+                       "intersect_group": "intersect_group (GroupCountKernel)",
+                       "intersect_lane": "intersect_lane (PrepCountKernel)"}[fam],
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -260,6 +288,10 @@ def main():
             "traffic": None,
             "avg_launch_ms": avg_ms,
             "launches": dom["launches"],
+            "families": {
+                "intersect_all": family_roofline(("intersect", "intersect_group", "intersect_lane")),
+                "scatter": family_roofline(("expand", "compact")),
+            },
         },
     }
     # HBM bytes per launch measured by rocprofv3 counters for this same

@@ -77,6 +77,7 @@ typedef struct {
   uint64_t d2h_syncs;        /* blocking device->host reads */
   uint64_t peak_pool_bytes;  /* high-water mark of the intermediate pool */
   double elapsed_ms;         /* host wall time of the call */
+  uint64_t ranked;           /* 1: the call ran on the degree-ranked copy */
 } sme_stats;
 
 /* ---- context ---------------------------------------------------------- */
@@ -102,6 +103,19 @@ void sme_graph_destroy(sme_graph* g);
  * binary-searches (gpu/utils/ops/GPUBinSearch.cuh:18-31). */
 int sme_set_hub_bitmaps(sme_ctx* ctx, uint64_t min_degree, uint64_t max_bytes);
 int sme_graph_hubs(const sme_graph* g, uint64_t* nhub);
+/* Degree-ranked copy, built by later uploads when on: the same graph with
+ * vertex ids reassigned in increasing (degree, id) order and every list
+ * re-sorted (one more CSR on the device).  sme_count over all start
+ * vertices runs on the copy when the answer cannot differ: a clique plan
+ * (Q0, Q3, Q7, Q11), whose count and per-level outputs are all invariant
+ * under relabeling, or any plan when level_counts is NULL (a count is
+ * invariant).  Listing each instance from its lowest-degree vertex bounds
+ * the lists walked (Chiba-Nishizeki orientation).  Start-range, bucketed
+ * and match-list calls use the original ids.  Off by default.  No
+ * counterpart in the reference, which lists from the lowest id
+ * (LazyTraversalPlan.h:27-28). */
+int sme_set_rank_order(sme_ctx* ctx, int on);
+int sme_graph_ranked(const sme_graph* g, int* on);
 
 /* ---- plan (host only) ------------------------------------------------- */
 int sme_plan_create(int algo, int query_type, int compress_level, int enable_ordering,
@@ -126,6 +140,17 @@ void sme_plan_destroy(sme_plan* p);
  * COMPUTE_COUNT. */
 int sme_count(sme_ctx* ctx, const sme_graph* g, const sme_plan* p, uint32_t v_begin,
               uint32_t v_end, uint64_t* out_count, uint64_t* level_counts, sme_stats* stats);
+
+/* sme_count over a start range [r_begin, r_end) of the graph's degree-ranked
+ * copy (sme_set_rank_order), in that copy's ids: the tasks of any partition
+ * of [0, n) sum to the full count (and, for cliques, to the full level
+ * outputs), so a long count can run as several calls.  SME_ERR_INVALID when
+ * the graph has no ranked copy.  No counterpart in the reference; it plays
+ * the role of IntraPartTask ranges (src/gpu/task/Task.h:23-43) on the
+ * relabeled graph. */
+int sme_count_ranked(sme_ctx* ctx, const sme_graph* g, const sme_plan* p, uint32_t r_begin,
+                     uint32_t r_end, uint64_t* out_count, uint64_t* level_counts,
+                     sme_stats* stats);
 
 /* One enumeration over [bounds[0], bounds[nbuckets]) that also splits the
  * count by start vertex: counts[k] receives the embeddings whose first
@@ -175,6 +200,12 @@ int sme_plan_vertices(const sme_plan* p);
  *                  times as many, the list is read in order as the pivot and
  *                  the segment is probed (from LDS) instead of probing the
  *                  list at random (0: the shorter is the pivot).
+ *   group_max      counting levels that extend a reused candidate segment
+ *                  with one adjacency list: the paths sharing a segment of
+ *                  at most group_max values (capped at 512) are counted by
+ *                  one wavefront that keeps the segment in LDS and reads
+ *                  each path's list in order, 64 consecutive entries at a
+ *                  time (0: off, the default).
  * Settings apply to every later call on the context; results never depend
  * on them. */
 typedef struct {
@@ -183,7 +214,7 @@ typedef struct {
   uint32_t super_chunks;
   uint64_t batch_bytes;
   uint32_t stream_ratio;
-  uint32_t reserved;
+  uint32_t group_max;
 } sme_tuning;
 int sme_set_tuning(sme_ctx* ctx, const sme_tuning* t);
 int sme_get_tuning(const sme_ctx* ctx, sme_tuning* t);

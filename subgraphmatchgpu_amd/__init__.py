@@ -49,18 +49,19 @@ class SmeError(RuntimeError):
 
 class _Stats(ctypes.Structure):
     _fields_ = [("batches", ctypes.c_uint64), ("d2h_syncs", ctypes.c_uint64),
-                ("peak_pool_bytes", ctypes.c_uint64), ("elapsed_ms", ctypes.c_double)]
+                ("peak_pool_bytes", ctypes.c_uint64), ("elapsed_ms", ctypes.c_double),
+                ("ranked", ctypes.c_uint64)]
 
 
 class _Tuning(ctypes.Structure):
     _fields_ = [("lane_path_max", ctypes.c_uint64), ("item_grid", ctypes.c_uint32),
                 ("super_chunks", ctypes.c_uint32), ("batch_bytes", ctypes.c_uint64),
-                ("stream_ratio", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+                ("stream_ratio", ctypes.c_uint32), ("group_max", ctypes.c_uint32)]
 
     def as_dict(self) -> dict:
         return {"lane_path_max": int(self.lane_path_max), "item_grid": int(self.item_grid),
                 "super_chunks": int(self.super_chunks), "batch_bytes": int(self.batch_bytes),
-                "stream_ratio": int(self.stream_ratio)}
+                "stream_ratio": int(self.stream_ratio), "group_max": int(self.group_max)}
 
 
 class _Trial(ctypes.Structure):
@@ -134,6 +135,10 @@ def library() -> ctypes.CDLL:
         "sme_count_buckets": ([vp, vp, vp, vp, ctypes.c_uint64, vp, u64p, u64p, ctypes.POINTER(_Stats)],
                               ctypes.c_int),
         "sme_graph_hubs": ([vp, u64p], ctypes.c_int),
+        "sme_set_rank_order": ([vp, ctypes.c_int], ctypes.c_int),
+        "sme_count_ranked": ([vp, vp, vp, ctypes.c_uint32, ctypes.c_uint32, u64p, u64p,
+                              ctypes.POINTER(_Stats)], ctypes.c_int),
+        "sme_graph_ranked": ([vp, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
         "sme_set_tuning": ([vp, ctypes.POINTER(_Tuning)], ctypes.c_int),
         "sme_list_matches": ([vp, vp, vp, ctypes.c_uint32, ctypes.c_uint32, vp, ctypes.c_uint64, u64p, u64p],
                              ctypes.c_int),
@@ -302,6 +307,7 @@ class CountResult:
     d2h_syncs: int
     peak_pool_bytes: int
     elapsed_ms: float
+    ranked: bool = False
 
 
 class Device:
@@ -328,7 +334,36 @@ class Device:
         _check(lib.sme_count(self._h, dg._h, plan._h, v_begin, v_end, ctypes.byref(cnt), lv,
                              ctypes.byref(st)), self._h)
         return CountResult(int(cnt.value), [int(lv[i]) for i in range(plan.levels)], int(st.batches),
-                           int(st.d2h_syncs), int(st.peak_pool_bytes), float(st.elapsed_ms))
+                           int(st.d2h_syncs), int(st.peak_pool_bytes), float(st.elapsed_ms),
+                           bool(st.ranked))
+
+    def count_ranked(self, dg: "DeviceGraph", plan: Plan, r_begin: int, r_end: int,
+                     levels: bool = True) -> CountResult:
+        """Count over start range [r_begin, r_end) of the degree-ranked copy
+        (ids of that copy); the tasks of a partition of [0, n) sum to the
+        full count.  levels=False asks for no level outputs."""
+        lib = library()
+        cnt = ctypes.c_uint64()
+        lv = (ctypes.c_uint64 * MAX_LEVELS)()
+        st = _Stats()
+        _check(lib.sme_count_ranked(self._h, dg._h, plan._h, r_begin, r_end, ctypes.byref(cnt),
+                                    lv if levels else None, ctypes.byref(st)), self._h)
+        return CountResult(int(cnt.value), [int(lv[i]) for i in range(plan.levels)] if levels else [],
+                           int(st.batches), int(st.d2h_syncs), int(st.peak_pool_bytes),
+                           float(st.elapsed_ms), bool(st.ranked))
+
+    def count_only(self, dg: "DeviceGraph", plan: Plan, v_begin: int = 0, v_end: int | None = None) -> CountResult:
+        """sme_count without level outputs (a full-range count may then run
+        on the degree-ranked copy)."""
+        lib = library()
+        if v_end is None:
+            v_end = dg.n
+        cnt = ctypes.c_uint64()
+        st = _Stats()
+        _check(lib.sme_count(self._h, dg._h, plan._h, v_begin, v_end, ctypes.byref(cnt), None,
+                             ctypes.byref(st)), self._h)
+        return CountResult(int(cnt.value), [], int(st.batches), int(st.d2h_syncs),
+                           int(st.peak_pool_bytes), float(st.elapsed_ms), bool(st.ranked))
 
     def count_buckets(self, dg: "DeviceGraph", plan: Plan, bounds) -> tuple:
         """One pass over [bounds[0], bounds[-1]); returns (per-bucket counts
@@ -369,6 +404,11 @@ class Device:
         uploaded afterwards (0 disables)."""
         _check(library().sme_set_hub_bitmaps(self._h, min_degree, max_bytes), self._h)
 
+    def set_rank_order(self, on: bool = True):
+        """Graphs uploaded afterwards keep a degree-ranked copy on which
+        clique plans over all start vertices run (same count and levels)."""
+        _check(library().sme_set_rank_order(self._h, int(on)), self._h)
+
     def list_matches(self, dg: "DeviceGraph", plan: Plan, v_begin: int = 0, v_end: int | None = None,
                      max_matches: int = 1 << 20):
         """Matches of a GPSM plan as an (rows, nv) uint32 array in
@@ -383,7 +423,7 @@ class Device:
         return out[:rows.value], int(total.value)
 
     def set_tuning(self, lane_path_max: int = 0, item_grid: int = 0, batch_bytes: int = 0,
-                   super_chunks: int = 0, stream_ratio: int = 0):
+                   super_chunks: int = 0, stream_ratio: int = 0, group_max: int = 0):
         """Schedule tunables for later calls (results never depend on them):
         lane_path_max -- counting-level paths with at most this many pivot
         candidates are verified by one lane each; item_grid -- workgroup cap
@@ -392,8 +432,11 @@ class Device:
         super_chunks -- 64-candidate chunks per wavefront step (runs inside
         one path load the path once and stage a list window in LDS);
         stream_ratio -- read an adjacency list in order as the pivot when it
-        is at most this many times a short reused segment (0: off)."""
-        t = _Tuning(lane_path_max, item_grid, super_chunks, batch_bytes, stream_ratio, 0)
+        is at most this many times a short reused segment (0: off);
+        group_max -- counting levels that extend a reused segment with one
+        list count the paths sharing a segment of at most this many values
+        in one wavefront, segment in LDS, lists read in order (0: off)."""
+        t = _Tuning(lane_path_max, item_grid, super_chunks, batch_bytes, stream_ratio, group_max)
         _check(library().sme_set_tuning(self._h, ctypes.byref(t)), self._h)
 
     def get_tuning(self) -> dict:
@@ -458,6 +501,12 @@ class DeviceGraph:
         _check(library().sme_graph_hubs(self._h, ctypes.byref(c)))
         return c.value
 
+    @property
+    def ranked(self) -> bool:
+        c = ctypes.c_int()
+        _check(library().sme_graph_ranked(self._h, ctypes.byref(c)))
+        return bool(c.value)
+
     def release(self):
         if getattr(self, "_h", None):
             library().sme_graph_destroy(self._h)
@@ -485,7 +534,7 @@ class Server:
                             hub_min_degree, hub_max_bytes,
                             _Tuning(t.get("lane_path_max", 0), t.get("item_grid", 0),
                                     t.get("super_chunks", 0), t.get("batch_bytes", 0),
-                                    t.get("stream_ratio", 0), 0))
+                                    t.get("stream_ratio", 0), t.get("group_max", 0)))
         self._rp = np.ascontiguousarray(g.row_ptrs, dtype=np.uint64)
         self._cols = np.ascontiguousarray(g.cols, dtype=np.uint32)
         h = ctypes.c_void_p()

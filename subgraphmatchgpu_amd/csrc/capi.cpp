@@ -213,6 +213,27 @@ int sme_count(sme_ctx* ctx, const sme_graph* g, const sme_plan* p, uint32_t v_be
       stats->d2h_syncs = rs.d2h_syncs;
       stats->peak_pool_bytes = rs.peak_pool_bytes;
       stats->elapsed_ms = rs.elapsed_ms;
+      stats->ranked = rs.ranked;
+    }
+    return SME_OK;
+  });
+}
+
+int sme_count_ranked(sme_ctx* ctx, const sme_graph* g, const sme_plan* p, uint32_t r_begin,
+                     uint32_t r_end, uint64_t* out_count, uint64_t* level_counts, sme_stats* stats) {
+  return Guarded(ctx, [&]() -> int {
+    if (!ctx || !g || !p || !out_count) return Report(ctx, SME_ERR_INVALID, "null argument");
+    std::string err;
+    sme::RunStats rs;
+    const sme::Status s = sme::RunPlan(&ctx->impl, &g->impl, &p->plan, r_begin, r_end, out_count,
+                                       level_counts, &rs, &err, true);
+    if (s != sme::kOk) return Report(ctx, s, err);
+    if (stats) {
+      stats->batches = rs.batches;
+      stats->d2h_syncs = rs.d2h_syncs;
+      stats->peak_pool_bytes = rs.peak_pool_bytes;
+      stats->elapsed_ms = rs.elapsed_ms;
+      stats->ranked = rs.ranked;
     }
     return SME_OK;
   });
@@ -234,6 +255,7 @@ int sme_count_buckets(sme_ctx* ctx, const sme_graph* g, const sme_plan* p, const
       stats->d2h_syncs = rs.d2h_syncs;
       stats->peak_pool_bytes = rs.peak_pool_bytes;
       stats->elapsed_ms = rs.elapsed_ms;
+      stats->ranked = rs.ranked;
     }
     return SME_OK;
   });
@@ -260,6 +282,7 @@ int sme_set_tuning(sme_ctx* ctx, const sme_tuning* t) {
   ctx->impl.tuning.super_chunks = t->super_chunks;
   ctx->impl.tuning.batch_bytes = t->batch_bytes;
   ctx->impl.tuning.stream_ratio = t->stream_ratio;
+  ctx->impl.tuning.group_max = t->group_max;
   return SME_OK;
 }
 
@@ -271,6 +294,7 @@ int sme_get_tuning(const sme_ctx* ctx, sme_tuning* t) {
   t->super_chunks = ctx->impl.tuning.super_chunks;
   t->batch_bytes = ctx->impl.tuning.batch_bytes;
   t->stream_ratio = ctx->impl.tuning.stream_ratio;
+  t->group_max = ctx->impl.tuning.group_max;
   return SME_OK;
 }
 
@@ -300,35 +324,50 @@ int sme_autotune(sme_ctx* ctx, const sme_graph* g, const sme_plan* const* plans,
     sme::Context& c = ctx->impl;
     std::string err;
     std::vector<uint64_t> counts;
+    // one setting: the sample runs kRepeats times, the median total counts
+    constexpr int kRepeats = 3;
     auto measure = [&](const sme::Context::Tuning& t, double* ms) -> int {
       c.tuning = t;
-      double total = 0;
-      for (int i = 0; i < nplans; ++i) {
-        for (const auto& blk : blocks) {
-          uint64_t cnt = 0;
-          sme::RunStats rs;
-          sme::Status s;
-          if (bounds) {
-            counts.assign(blk.second - blk.first, 0);
-            s = sme::RunPlanBuckets(&c, &g->impl, &plans[i]->plan, bounds + blk.first,
-                                    blk.second - blk.first, counts.data(), &cnt, nullptr, &rs, &err);
-          } else {
-            s = sme::RunPlan(&c, &g->impl, &plans[i]->plan, static_cast<uint32_t>(blk.first),
-                             static_cast<uint32_t>(blk.second), &cnt, nullptr, &rs, &err);
+      double reps[kRepeats];
+      for (int rep = 0; rep < kRepeats; ++rep) {
+        double total = 0;
+        for (int i = 0; i < nplans; ++i) {
+          for (const auto& blk : blocks) {
+            uint64_t cnt = 0;
+            sme::RunStats rs;
+            sme::Status s;
+            if (bounds) {
+              counts.assign(blk.second - blk.first, 0);
+              s = sme::RunPlanBuckets(&c, &g->impl, &plans[i]->plan, bounds + blk.first,
+                                      blk.second - blk.first, counts.data(), &cnt, nullptr, &rs, &err);
+            } else {
+              s = sme::RunPlan(&c, &g->impl, &plans[i]->plan, static_cast<uint32_t>(blk.first),
+                               static_cast<uint32_t>(blk.second), &cnt, nullptr, &rs, &err);
+            }
+            if (s != sme::kOk) return Report(ctx, s, err);
+            total += rs.elapsed_ms;
           }
-          if (s != sme::kOk) return Report(ctx, s, err);
-          total += rs.elapsed_ms;
         }
+        reps[rep] = total;
       }
-      *ms = total;
+      std::sort(reps, reps + kRepeats);
+      *ms = reps[kRepeats / 2];
       return SME_OK;
+    };
+    auto to_c = [](const sme::Context::Tuning& t) {
+      return sme_tuning{t.lane_path_max, t.item_grid, t.super_chunks, t.batch_bytes,
+                        t.stream_ratio, t.group_max};
+    };
+    auto same = [](const sme::Context::Tuning& a, const sme::Context::Tuning& b) {
+      return a.lane_path_max == b.lane_path_max && a.item_grid == b.item_grid &&
+             a.super_chunks == b.super_chunks && a.batch_bytes == b.batch_bytes &&
+             a.stream_ratio == b.stream_ratio && a.group_max == b.group_max;
     };
     const sme::Context::Tuning start = c.tuning;
     int n = 0;
     auto record = [&](const sme::Context::Tuning& t, double ms) {
       if (trials && n < cap) {
-        trials[n].tuning = sme_tuning{t.lane_path_max, t.item_grid, t.super_chunks, t.batch_bytes,
-                                      t.stream_ratio, 0};
+        trials[n].tuning = to_c(t);
         trials[n].ms = ms;
       }
       ++n;
@@ -348,26 +387,28 @@ int sme_autotune(sme_ctx* ctx, const sme_graph* g, const sme_plan* const* plans,
       return rc;
     }
     record(cur, cur_ms);
+    // a setting replaces the current one only when its median is at least
+    // 2 % faster, so near-ties keep the earlier (simpler) choice
+    constexpr double kGain = 0.98;
     const uint64_t lane_opts[] = {0, 8, 32, 128, 512};
     const uint32_t super_opts[] = {0, 2, 4, 8, 16};
     const uint32_t grid_opts[] = {0, 8192, 4096, 2048, 1024};
     const uint64_t batch_opts[] = {0, 4ull << 30, 1ull << 30, 256ull << 20, 64ull << 20};
     const uint32_t stream_opts[] = {0, 4, 8, 16, 32};
-    for (int dim = 0; dim < 5; ++dim) {
-      const int nopt = 5;
+    const uint32_t group_opts[] = {0, 128, 512};
+    const int nopts[] = {5, 5, 5, 5, 5, 3};
+    for (int dim = 0; dim < 6; ++dim) {
       sme::Context::Tuning dim_best = cur;
       double dim_ms = cur_ms;
-      for (int k = 0; k < nopt; ++k) {
+      for (int k = 0; k < nopts[dim]; ++k) {
         sme::Context::Tuning t = cur;
         if (dim == 0) t.lane_path_max = lane_opts[k];
         if (dim == 1) t.super_chunks = super_opts[k];
         if (dim == 2) t.item_grid = grid_opts[k];
         if (dim == 3) t.batch_bytes = batch_opts[k];
         if (dim == 4) t.stream_ratio = stream_opts[k];
-        if (t.lane_path_max == cur.lane_path_max && t.item_grid == cur.item_grid &&
-            t.super_chunks == cur.super_chunks && t.batch_bytes == cur.batch_bytes &&
-            t.stream_ratio == cur.stream_ratio)
-          continue;
+        if (dim == 5) t.group_max = group_opts[k];
+        if (same(t, cur)) continue;
         double t_ms = 0;
         rc = measure(t, &t_ms);
         if (rc != SME_OK) {
@@ -375,7 +416,7 @@ int sme_autotune(sme_ctx* ctx, const sme_graph* g, const sme_plan* const* plans,
           return rc;
         }
         record(t, t_ms);
-        if (t_ms < dim_ms) {
+        if (t_ms < dim_ms * kGain) {
           dim_best = t;
           dim_ms = t_ms;
         }
@@ -384,8 +425,7 @@ int sme_autotune(sme_ctx* ctx, const sme_graph* g, const sme_plan* const* plans,
       cur_ms = dim_ms;
     }
     c.tuning = cur;
-    *best = sme_tuning{cur.lane_path_max, cur.item_grid, cur.super_chunks, cur.batch_bytes,
-                       cur.stream_ratio, 0};
+    *best = to_c(cur);
     if (best_ms) *best_ms = cur_ms;
     if (ntrials) *ntrials = n;
     return SME_OK;
@@ -412,6 +452,7 @@ int sme_server_open(int device, const uint64_t* row_ptrs, const uint32_t* cols, 
     c.tuning.super_chunks = cfg->tuning.super_chunks;
     c.tuning.batch_bytes = cfg->tuning.batch_bytes;
     c.tuning.stream_ratio = cfg->tuning.stream_ratio;
+    c.tuning.group_max = cfg->tuning.group_max;
     std::string err;
     const sme::Status st = s->impl.Open(device, row_ptrs, cols, n, nnz, c, &err);
     if (st != sme::kOk) {
@@ -467,6 +508,18 @@ int sme_set_hub_bitmaps(sme_ctx* ctx, uint64_t min_degree, uint64_t max_bytes) {
 int sme_graph_hubs(const sme_graph* g, uint64_t* nhub) {
   if (!g || !nhub) return SME_ERR_INVALID;
   *nhub = g->impl.nhub;
+  return SME_OK;
+}
+
+int sme_set_rank_order(sme_ctx* ctx, int on) {
+  if (!ctx) return SME_ERR_INVALID;
+  ctx->impl.rank_order = on != 0;
+  return SME_OK;
+}
+
+int sme_graph_ranked(const sme_graph* g, int* on) {
+  if (!g || !on) return SME_ERR_INVALID;
+  *on = g->impl.ranked ? 1 : 0;
   return SME_OK;
 }
 

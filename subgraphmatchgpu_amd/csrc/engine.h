@@ -34,6 +34,7 @@ enum KernelSlot : int {
   kSlotCount,       // COUNT kernels
   kSlotCompact,     // candidate compaction + segment windows
   kSlotExpand,      // MATERIALIZE scatter of partial instances
+  kSlotGroup,       // COMPUTE_COUNT of paths sharing a staged segment
   kSlotMisc,
   kNumSlots
 };
@@ -51,6 +52,7 @@ struct RunStats {
   uint64_t d2h_syncs = 0;
   uint64_t peak_pool_bytes = 0;
   double elapsed_ms = 0;
+  uint64_t ranked = 0;  // 1: ran on the degree-ranked copy of the graph
 };
 
 class Context {
@@ -81,6 +83,9 @@ class Context {
   // (0 disables)
   uint64_t hub_min_degree = 0;
   uint64_t hub_max_bytes = 0;
+  // graphs uploaded from now on also keep a degree-ranked copy (RankByDegree)
+  // on which clique plans over all start vertices run
+  bool rank_order = false;
 
   // Schedule tunables (chosen by hand or by the autotuner):
   //   lane_path_max  COMPUTE_COUNT paths with at most this many pivot items
@@ -95,12 +100,18 @@ class Context {
   //                  share of the pool)
   //   stream_ratio   pivot rule for a reused segment plus one adjacency
   //                  list (see PrepComputeArgs; 0: shortest list)
+  //   group_max      counting levels that extend a reused segment with one
+  //                  adjacency list: paths sharing a segment of at most this
+  //                  many values are counted together by one wave that keeps
+  //                  the segment in LDS and streams each path's list
+  //                  (GroupCountKernel); 0: off
   struct Tuning {
     uint64_t lane_path_max = 0;
     uint32_t item_grid = 0;
     uint32_t super_chunks = 4;
     uint64_t batch_bytes = 0;
     uint32_t stream_ratio = 0;
+    uint32_t group_max = 0;
   } tuning;
 
   // profiling
@@ -132,11 +143,17 @@ class Context {
 class DevGraph {
  public:
   ~DevGraph();
+  // Copies the CSR (plus hub bitmaps and, with ctx->rank_order, the
+  // degree-ranked copy as configured on the context).
   Status Upload(Context* ctx, const uint64_t* row_ptrs, const uint32_t* cols, uint64_t n,
                 uint64_t nnz, std::string* err);
   uint64_t n = 0, nnz = 0;
   uint64_t* rp = nullptr;
   uint32_t* cols = nullptr;
+  // up[v] = index in cols of v's first neighbour above v (rp[v + 1] if none)
+  uint64_t* up = nullptr;
+  // isomorphic degree-ranked copy (RankByDegree), or nullptr
+  DevGraph* ranked = nullptr;
   bool self_loops = false;
   int device = -1;
   // hub adjacency bitmaps (nhub rows of hub_words u32 words)
@@ -146,10 +163,14 @@ class DevGraph {
 };
 
 // Runs one plan over start vertices [v_begin, v_end) of the first matched
-// vertex (an IntraPartTask, src/gpu/task/Task.h:23-43).
+// vertex (an IntraPartTask, src/gpu/task/Task.h:23-43).  A run over all
+// start vertices goes to the degree-ranked copy when the graph has one and
+// the answer cannot differ there: a clique plan, or no level outputs asked
+// for.  With `ranked_ids` the range is in the ranked copy's ids (a task of a
+// partition of [0, n) over that copy; the copy must exist).
 Status RunPlan(Context* ctx, const DevGraph* g, const EnginePlan* plan, uint32_t v_begin,
                uint32_t v_end, uint64_t* count, uint64_t* level_counts, RunStats* stats,
-               std::string* err);
+               std::string* err, bool ranked_ids = false);
 
 // Same walk over [bounds[0], bounds[nbuckets]); additionally counts[k]
 // receives the embeddings whose first matched vertex lies in

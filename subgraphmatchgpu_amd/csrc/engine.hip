@@ -25,7 +25,7 @@ void Fail(Status s, const std::string& msg) { throw Error{s, msg}; }
 const char* SlotName(int slot) {
   static const char* kNames[kNumSlots] = {"prep",  "scan",   "partition", "intersect",
                                           "intersect_lane", "filter", "count", "compact",
-                                          "expand", "misc"};
+                                          "expand", "intersect_group", "misc"};
   return slot >= 0 && slot < kNumSlots ? kNames[slot] : "?";
 }
 
@@ -134,6 +134,8 @@ DevGraph::~DevGraph() {
   if (cols) (void)hipFree(cols);
   if (hub_id) (void)hipFree(hub_id);
   if (hub_bits) (void)hipFree(hub_bits);
+  if (up) (void)hipFree(up);
+  delete ranked;
 }
 
 Status DevGraph::Upload(Context* ctx, const uint64_t* row_ptrs, const uint32_t* c, uint64_t nv,
@@ -195,6 +197,27 @@ Status DevGraph::Upload(Context* ctx, const uint64_t* row_ptrs, const uint32_t* 
       SME_HIP(hipStreamSynchronize(ctx->stream()));
       (void)hipFree(hubs);
     }
+  }
+  if (hipMalloc(reinterpret_cast<void**>(&up), (nv ? nv : 1) * sizeof(uint64_t)) != hipSuccess) {
+    (void)hipGetLastError();
+    *err = "hipMalloc of the list split points failed";
+    return kErrOom;
+  }
+  LaunchUpperStart(rp, cols, nv, up, ctx->stream());
+  SME_HIP(hipStreamSynchronize(ctx->stream()));
+  if (ctx->rank_order && nv) {
+    std::vector<uint64_t> rrp;
+    std::vector<uint32_t> rcols, new_id;
+    RankByDegree(nv, row_ptrs, c, &rrp, &rcols, &new_id);
+    ranked = new DevGraph;
+    // the copy has no copy of its own (restored on every exit)
+    struct Restore {
+      Context* c;
+      ~Restore() { c->rank_order = true; }
+    } restore{ctx};
+    ctx->rank_order = false;
+    const Status s = ranked->Upload(ctx, rrp.data(), rcols.data(), nv, ne, err);
+    if (s != kOk) return s;
   }
   return kOk;
 }
@@ -311,8 +334,9 @@ class Runner {
 
   void Run(uint32_t v_begin, uint32_t v_end, uint64_t* count, uint64_t* levels, RunStats* stats) {
     Analyze();
-    // [total, per-level counts, per-level items counted by lanes]
-    const int nacc = 1 + 2 * plan_->nops;
+    // [total, per-level counts, per-level items counted by lanes, per-level
+    // values streamed by group counting, per-level group walk lower bound]
+    const int nacc = 1 + 4 * plan_->nops;
     Buf<unsigned long long> acc(pool_, nacc);
     acc_ = acc.get();
     SME_HIP(hipMemsetAsync(acc_, 0, sizeof(unsigned long long) * nacc, st_));
@@ -320,7 +344,7 @@ class Runner {
     std::memset(host_level_, 0, sizeof(host_level_));
     host_count_ = 0;
     if (v_end > v_begin) Level0(v_begin, v_end);
-    unsigned long long dev[1 + 2 * kMaxOps];
+    unsigned long long dev[1 + 4 * kMaxOps];
     SME_HIP(hipMemcpyAsync(ctx_->pinned(), acc_, sizeof(unsigned long long) * nacc,
                            hipMemcpyDeviceToHost, st_));
     SME_HIP(hipStreamSynchronize(st_));
@@ -333,6 +357,13 @@ class Runner {
         const uint64_t items = dev[1 + plan_->nops + L];
         ls.bytes += 4.0 * lane_lists_[L] * static_cast<double>(items);
         ls.items += items;
+      }
+      // group counting: 4 bytes per list (two) per item of the shorter
+      // window, the walk of a pivot-and-probe intersection
+      SlotStats& gs = ctx_->slots[kSlotGroup];
+      for (int L = 0; L < plan_->nops; ++L) {
+        gs.items += dev[1 + 2 * plan_->nops + L];
+        gs.bytes += 8.0 * static_cast<double>(dev[1 + 3 * plan_->nops + L]);
       }
     }
     uint64_t total = host_count_ + dev[0];
@@ -725,9 +756,39 @@ class Runner {
     const int u = op.u;
     const bool zero_copy = nlists == 1 && nne == 0 && !pair;
     const double prep_bytes = P * (4.0 * (nlists + pa.nlt + pa.ngt) + 16.0 * nlists + 8.0 + 17.0);
+    // paths extending a shared segment with one list: one wave per group
+    const bool group = op.kind == kComputeCount && reuse && nb == 1 && !pair && nne <= 2 &&
+                       ctx_->tuning.group_max > 0;
+    if (group) {
+      pa.group_cap = std::min<uint32_t>(ctx_->tuning.group_max, kGroupMax);
+      GroupCountArgs ga{};
+      ga.npaths = P;
+      ga.cs = f.cs;
+      ga.seg = f.reuse;
+      ga.adj_v = lists[0];
+      ga.rp = g_->rp;
+      ga.cols = g_->cols;
+      ga.up = g_->up;
+      WindowConds(op, &ga.nlt, ga.lt_v, &ga.ngt, ga.gt_v);
+      ga.nne = nne;
+      ga.ne_v[0] = nne > 0 ? ne_v[0] : 0;
+      ga.ne_v[1] = nne > 1 ? ne_v[1] : 0;
+      ga.cap = pa.group_cap;
+      ga.acc = acc_;
+      ga.acc_level = acc_ + 1 + L;
+      if (ctx_->profiling) {
+        ga.walked = acc_ + 1 + 2 * plan_->nops + L;
+        ga.lower = acc_ + 1 + 3 * plan_->nops + L;
+      }
+      ga.bk = bk_;
+      // per path: gathered columns, two list bounds and the segment index;
+      // the per-item bytes are added from the device counters after the run
+      Timed t(ctx_, st_, kSlotGroup, P * (4.0 * (1 + ga.nlt + ga.ngt + nne) + 32.0 + 4.0), 0);
+      LaunchGroupCount(ga, ctx_->tuning.item_grid, st_);
+    }
     // short paths of a counting level: one lane per path in the set-up kernel
     const bool lane_count =
-        op.kind == kComputeCount && !zero_copy && ctx_->tuning.lane_path_max > 0;
+        op.kind == kComputeCount && !zero_copy && !group && ctx_->tuning.lane_path_max > 0;
     if (lane_count) {
       PrepCountArgs pc{};
       pc.pa = pa;
@@ -1245,9 +1306,13 @@ Status Recover(Context* ctx, Status s, const std::string& msg, std::string* err)
 
 Status RunPlan(Context* ctx, const DevGraph* g, const EnginePlan* plan, uint32_t v_begin,
                uint32_t v_end, uint64_t* count, uint64_t* level_counts, RunStats* stats,
-               std::string* err) {
+               std::string* err, bool ranked_ids) {
   if (!ctx || !g || !plan || !count) {
     *err = "null argument";
+    return kErrInvalid;
+  }
+  if (ranked_ids && !g->ranked) {
+    *err = "the graph has no degree-ranked copy (upload it with rank order on)";
     return kErrInvalid;
   }
   if (g->device != ctx->device()) {
@@ -1256,13 +1321,21 @@ Status RunPlan(Context* ctx, const DevGraph* g, const EnginePlan* plan, uint32_t
   }
   if (v_end > g->n) v_end = static_cast<uint32_t>(g->n);
   if (v_begin > v_end) v_begin = v_end;
+  // A count over all start vertices is the same on any isomorphic copy of
+  // the graph, and so are a clique plan's level outputs; the degree-ranked
+  // copy walks far shorter lists (each instance is reached from its
+  // lowest-degree vertex first).
+  const bool use_ranked =
+      ranked_ids || (g->ranked && v_begin == 0 && v_end == g->n && (plan->clique || !level_counts));
+  const DevGraph* rg = use_ranked ? g->ranked : g;
   const auto t0 = std::chrono::steady_clock::now();
   try {
     SME_HIP(hipSetDevice(ctx->device()));
     ctx->held_peak = ctx->held;
-    Runner r(ctx, g, plan);
+    Runner r(ctx, rg, plan);
     RunStats st;
     r.Run(v_begin, v_end, count, level_counts, &st);
+    st.ranked = use_ranked ? 1 : 0;
     if (ctx->profiling) ctx->ResolveMarks();
     st.peak_pool_bytes = ctx->held_peak;
     st.elapsed_ms =

@@ -366,4 +366,33 @@ void GenerateErdosRenyiBatch(uint64_t ngraphs, uint64_t n_min, uint64_t n_max, d
   g->row_ptrs[g->n] = g->nnz;
 }
 
+void RankByDegree(uint64_t n, const uintE* rp, const uintV* cols, std::vector<uintE>* rrp,
+                  std::vector<uintV>* rcols, std::vector<uintV>* new_id) {
+  // stable counting sort of the vertices by degree
+  uint64_t maxd = 0;
+  for (uint64_t v = 0; v < n; ++v) maxd = std::max<uint64_t>(maxd, rp[v + 1] - rp[v]);
+  std::vector<uint64_t> start(maxd + 2, 0);
+  for (uint64_t v = 0; v < n; ++v) ++start[rp[v + 1] - rp[v] + 1];
+  for (uint64_t d = 0; d <= maxd; ++d) start[d + 1] += start[d];
+  new_id->resize(n);
+  std::vector<uintV> old(n);
+  for (uint64_t v = 0; v < n; ++v) {
+    const uint64_t u = start[rp[v + 1] - rp[v]]++;
+    (*new_id)[v] = static_cast<uintV>(u);
+    old[u] = static_cast<uintV>(v);
+  }
+  rrp->assign(n + 1, 0);
+  for (uint64_t u = 0; u < n; ++u) (*rrp)[u + 1] = (*rrp)[u] + (rp[old[u] + 1] - rp[old[u]]);
+  rcols->resize(rp[n]);
+  const uintV* nid = new_id->data();
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (int64_t u = 0; u < static_cast<int64_t>(n); ++u) {
+    const uint64_t v = old[u];
+    uintV* out = rcols->data() + (*rrp)[u];
+    const uint64_t d = rp[v + 1] - rp[v];
+    for (uint64_t i = 0; i < d; ++i) out[i] = nid[cols[rp[v] + i]];
+    std::sort(out, out + d);
+  }
+}
+
 }  // namespace sme

@@ -60,6 +60,16 @@ void GenerateErdosRenyiBatch(uint64_t ngraphs, uint64_t n_min, uint64_t n_max, d
                              double deg_max, uint64_t seed, HostGraph* g,
                              std::vector<uint64_t>* graph_offs);
 
+// Degree-ranked copy of a CSR: vertex v becomes new_id[v], its position in
+// increasing (degree, id) order, and every list is mapped and re-sorted.  The
+// copy is isomorphic to the input, so a pattern counted over all start
+// vertices has the same count; listing a clique from its lowest id then
+// starts at its lowest-degree vertex, and the lists "above" a vertex hold
+// only neighbours of at least its degree (the orientation of
+// Chiba-Nishizeki listing).
+void RankByDegree(uint64_t n, const uintE* row_ptrs, const uintV* cols, std::vector<uintE>* rrp,
+                  std::vector<uintV>* rcols, std::vector<uintV>* new_id);
+
 // Deterministic 64-bit PRNG pair used by the generators.
 struct SplitMix64 {
   uint64_t s;

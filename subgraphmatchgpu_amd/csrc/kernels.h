@@ -65,6 +65,9 @@ struct PrepComputeArgs {
   // is what gets probed (staged in LDS by the verify fast path); 0: the
   // shortest list is the pivot.
   uint32_t stream_ratio;
+  // paths whose reused segment holds at most group_cap values are counted
+  // by GroupCountKernel: they get no items here (0: none)
+  uint32_t group_cap;
 };
 
 constexpr uint32_t kStageEntries = 1024;  // LDS entries per wave for a staged list
@@ -96,6 +99,40 @@ struct BucketRef {
   const uint32_t* table;
   uint32_t base;
 };
+
+// COMPUTE_COUNT of paths that extend a reused parent segment S with one
+// adjacency list N(M[adj_v]) (the last level of a clique, for instance).
+// Consecutive paths with the same segment -- the children of one parent --
+// form a group.  One wave stages S in LDS once per group (sorted, plus a
+// hash table of it), then streams every child's window of
+// N(M[adj_v]) in order, 64 consecutive entries per load, and counts the
+// values found in S.  Reads of the graph are coalesced instead of one
+// random line per membership probe.  Paths whose segment holds more than
+// `cap` values are left to the item schedule.
+struct GroupCountArgs {
+  uint64_t npaths;
+  ColSet cs;
+  CandRef seg;
+  uint8_t adj_v;
+  const uint64_t* rp;
+  const uint32_t* cols;
+  const uint64_t* up;  // first neighbour above v (nullptr: binary search)
+  uint8_t nlt, ngt;
+  uint8_t lt_v[kQ];
+  uint8_t gt_v[kQ];
+  uint8_t nne;
+  uint8_t ne_v[2];
+  uint32_t cap;
+  unsigned long long* acc;
+  unsigned long long* acc_level;
+  // profiling (nullable): values streamed, and the sum over paths of
+  // min(|S in window|, |list window|) (the walk of a pivot-and-probe
+  // intersection, used for algorithmic bytes)
+  unsigned long long* walked;
+  unsigned long long* lower;
+  BucketRef bk;
+};
+constexpr uint32_t kGroupMax = 512;  // longest segment a wave stages
 
 // Item kernels.  `offs` / per-segment arrays are already offset to the
 // batch's first segment; items are numbered from offs[0].
@@ -245,6 +282,10 @@ void LaunchChunkPartition(const uint64_t* offs, uint64_t nseg, uint64_t nchunks,
 // loops over chunks, so any cap >= 1 covers all items.
 void LaunchVerify(const ItemArgs& a, int mode, uint32_t max_grid, hipStream_t stream);
 void LaunchCompact(const CompactArgs& a, hipStream_t stream);
+void LaunchGroupCount(const GroupCountArgs& a, uint32_t max_grid, hipStream_t stream);
+// up[v] = first index of rp[v]..rp[v+1] holding a neighbour > v.
+void LaunchUpperStart(const uint64_t* rp, const uint32_t* cols, uint64_t n, uint64_t* up,
+                      hipStream_t stream);
 void LaunchExpand(const ExpandArgs& a, hipStream_t stream);
 constexpr uint32_t kNoHub = 0xffffffffu;
 // Sets bit x of row h for every x in N(hubs[h]); rows must be zero.

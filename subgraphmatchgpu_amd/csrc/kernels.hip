@@ -244,6 +244,12 @@ __device__ __forceinline__ void PrepPath(const PrepComputeArgs& a, uint64_t p, u
 __global__ __launch_bounds__(kThreads) void PrepComputeKernel(PrepComputeArgs a) {
   for (uint64_t p = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; p < a.npaths;
        p += GridStride()) {
+    if (a.group_cap && a.has_reuse && a.reuse.len[SegIndex(a.reuse, p)] <= a.group_cap) {
+      a.size[p] = 0;  // counted by GroupCountKernel
+      a.pstart[p] = 0;
+      if (a.piv) a.piv[p] = 0;
+      continue;
+    }
     uint64_t b, e;
     uint8_t pv;
     PrepPath(a, p, &b, &e, &pv);
@@ -1078,6 +1084,240 @@ void ScanImpl(const T* in, uint64_t n, uint64_t* out, uint64_t* tmp, hipStream_t
   hipLaunchKernelGGL(ScanTileApplyKernel<T>, dim3(tiles), dim3(kThreads), 0, stream, in, n, sums, o);
 }
 
+// --------------------------------------------------------------------------
+// GroupCountKernel (see GroupCountArgs).
+//
+// One wave per run of paths sharing a segment S.  S is staged in LDS as a
+// sorted array (set[0, n)) and a hash table of 4-slot buckets at set +
+// kGroupMax (at most two values per bucket on average).  A value goes to the
+// first free slot of its bucket, or of the next bucket when all four are
+// taken; a lookup reads a whole bucket with one 16-byte LDS load and moves
+// on only when that bucket is full and lacks the value, so lanes rarely
+// loop.  The children's list windows are cut into tiles of 64 consecutive
+// entries whose owner is wave-uniform, so per tile the owner's bounds are
+// scalar and the entries are one coalesced load.
+constexpr uint32_t kEmptySlot = 0xffffffffu;   // never a vertex id (n < 2^32 - 1)
+constexpr uint64_t kGroupUnit = 256;           // paths per work unit of a wave
+constexpr int kGroupUnroll = 8;                // tiles of 64 list entries in flight per wave
+constexpr uint32_t kGroupLdsWords = 3 * kGroupMax;  // sorted values + buckets (load <= 1/2)
+
+struct alignas(16) Bucket4 {
+  uint32_t s[4];
+};
+
+__device__ __forceinline__ uint32_t HashSlot(uint32_t x, uint32_t shift) {
+  return (x * 2654435761u) >> shift;  // Knuth's multiplicative hash
+}
+
+__device__ __forceinline__ void BucketInsert(uint32_t* tab, uint32_t shift, uint32_t mask, uint32_t x) {
+  uint32_t b = HashSlot(x, shift);
+  while (true) {
+    uint32_t* slot = tab + 4 * b;
+    for (int k = 0; k < 4; ++k)
+      if (atomicCAS(slot + k, kEmptySlot, x) == kEmptySlot) return;
+    b = (b + 1) & mask;
+  }
+}
+
+// Number of values of the sorted a[0, n) below key; the steps depend on n only.
+__device__ __forceinline__ uint32_t RankLds(const uint32_t* a, uint32_t n, uint64_t key) {
+  if (n == 0) return 0;
+  uint32_t b = 0, len = n;
+  while (len > 1) {
+    const uint32_t half = len >> 1;
+    b = a[b + half - 1] < key ? b + half : b;
+    len -= half;
+  }
+  return b + (a[b] < key ? 1u : 0u);
+}
+
+__device__ __forceinline__ uint32_t ReadLane32(uint32_t v, uint32_t lane) {
+  return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), static_cast<int>(lane)));
+}
+
+__device__ __forceinline__ uint64_t ReadLane64(uint64_t v, uint32_t lane) {
+  return (static_cast<uint64_t>(ReadLane32(static_cast<uint32_t>(v >> 32), lane)) << 32) |
+         ReadLane32(static_cast<uint32_t>(v), lane);
+}
+
+__device__ __forceinline__ uint32_t Scalar32(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__global__ __launch_bounds__(kThreads) void GroupCountKernel(GroupCountArgs a) {
+  __shared__ alignas(16) uint32_t lds_set[kWavesPerBlock][kGroupLdsWords];
+  const int lane = threadIdx.x & (kWave - 1);
+  uint32_t* set = lds_set[threadIdx.x / kWave];
+  const Bucket4* tab = reinterpret_cast<const Bucket4*>(set + kGroupMax);
+  const uint64_t wave = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) / kWave;
+  const uint64_t nwaves = GridStride() / kWave;
+  const uint64_t nunits = (a.npaths + kGroupUnit - 1) / kGroupUnit;
+  u64 local = 0, walked = 0, lower = 0;
+  BucketAcc bacc{kNoBucket, 0};
+  for (uint64_t unit = wave; unit < nunits; unit += nwaves) {
+    const uint64_t pb = unit * kGroupUnit;
+    const uint64_t pe = pb + kGroupUnit < a.npaths ? pb + kGroupUnit : a.npaths;
+    uint64_t staged = ~0ull;
+    uint32_t n = 0, shift = 0, mask = 0, smin = 0, smax = 0;  // wave-uniform
+    for (uint64_t sub = pb; sub < pe; sub += kWave) {
+      const uint64_t p = sub + lane;
+      const bool valid = p < pe;
+      const uint64_t seg = valid ? SegIndex(a.seg, p) : ~0ull;
+      u64 pending = __ballot(valid);
+      while (pending) {
+        // the run of paths sharing the lowest pending lane's segment (paths
+        // of one segment are consecutive)
+        const int f = __ffsll(static_cast<long long>(pending)) - 1;
+        const uint64_t s = ReadLane64(seg, static_cast<uint32_t>(f));
+        const bool in_run = valid && seg == s;
+        pending &= ~__ballot(in_run);
+        const uint64_t slen = a.seg.len[s];
+        if (slen == 0 || slen > a.cap) continue;
+        if (s != staged) {
+          const uint32_t* src = a.seg.base + a.seg.beg[s];
+          n = static_cast<uint32_t>(slen);
+          uint32_t bbits = 4;  // buckets: a power of two >= n / 2
+          while ((2u << bbits) < n) ++bbits;
+          shift = 32 - bbits;
+          mask = (1u << bbits) - 1;
+          WaveSync();  // earlier readers of the stage are done
+          for (uint32_t i = lane; i < 4 * (mask + 1); i += kWave) set[kGroupMax + i] = kEmptySlot;
+          WaveSync();
+          for (uint32_t i = lane; i < n; i += kWave) {
+            const uint32_t x = src[i];
+            set[i] = x;
+            BucketInsert(set + kGroupMax, shift, mask, x);
+          }
+          WaveSync();
+          smin = Scalar32(set[0]);
+          smax = Scalar32(set[n - 1]);
+          staged = s;
+        }
+        // each child path's window of its adjacency list
+        uint64_t wb = 0;
+        uint32_t clen = 0, ne0 = kEmptySlot, ne1 = kEmptySlot;
+        if (in_run) {
+          const uint32_t v = Gather(a.cs, a.adj_v, p);
+          uint64_t lo = smin, hi = static_cast<uint64_t>(smax) + 1;
+          for (int i = 0; i < a.ngt; ++i) {
+            const uint64_t g = Gather(a.cs, a.gt_v[i], p) + 1ull;
+            lo = g > lo ? g : lo;
+          }
+          for (int i = 0; i < a.nlt; ++i) {
+            const uint64_t l = Gather(a.cs, a.lt_v[i], p);
+            hi = l < hi ? l : hi;
+          }
+          if (lo < hi) {
+            const uint64_t rb = a.rp[v], re = a.rp[v + 1];
+            const uint64_t b =
+                (a.up && lo == static_cast<uint64_t>(v) + 1) ? a.up[v] : LowerBound(a.cols, rb, re, static_cast<uint32_t>(lo));
+            // without a < condition the list is streamed to its end: values
+            // above the segment's maximum fail the membership test
+            const uint64_t e = a.nlt ? LowerBound(a.cols, b, re, static_cast<uint32_t>(hi)) : re;
+            wb = b;
+            clen = static_cast<uint32_t>(e - b);
+          }
+          if (a.nne > 0) ne0 = Gather(a.cs, a.ne_v[0], p);
+          if (a.nne > 1) ne1 = Gather(a.cs, a.ne_v[1], p);
+          if (a.lower && lo < hi) {
+            const uint32_t sw = RankLds(set, n, hi) - RankLds(set, n, lo);
+            lower += sw < clen ? sw : clen;
+          }
+        }
+        // tiles of 64 consecutive entries: the owner of tile t is the
+        // number of children whose tiles all lie before t, found by one
+        // ballot, so per tile the owner's bounds are scalar
+        const uint32_t own_tiles = (clen + kWave - 1) / kWave;
+        uint32_t incl = own_tiles;
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+          const uint32_t t = __shfl_up(incl, o);
+          if (lane >= o) incl += t;
+        }
+        const uint32_t excl = incl - own_tiles;
+        const uint32_t ntiles = ReadLane32(incl, kWave - 1);
+        uint32_t hits = 0;
+        for (uint32_t tb = 0; tb < ntiles; tb += kGroupUnroll) {
+          uint32_t y[kGroupUnroll], c[kGroupUnroll];
+          bool ok[kGroupUnroll];
+#pragma unroll
+          for (int u = 0; u < kGroupUnroll; ++u) {  // issue every load first
+            const uint32_t tt = tb + u;
+            const uint32_t cu = static_cast<uint32_t>(__popcll(__ballot(incl <= tt)));
+            c[u] = cu < kWave - 1 ? cu : kWave - 1;
+            const uint32_t off = (tt - ReadLane32(excl, c[u])) * kWave + lane;
+            const uint32_t len = ReadLane32(clen, c[u]);
+            const uint64_t base = ReadLane64(wb, c[u]);
+            ok[u] = tt < ntiles && off < len;
+            // a lane past the window (or a tile past the last) reads a
+            // valid entry and is discarded below
+            y[u] = a.cols[base + (ok[u] ? off : 0u)];
+          }
+#pragma unroll
+          for (int u = 0; u < kGroupUnroll; ++u) {
+            uint32_t v = ok[u] ? y[u] : kEmptySlot;
+            if (a.nne > 0) v = v == ReadLane32(ne0, c[u]) ? kEmptySlot : v;
+            if (a.nne > 1) v = v == ReadLane32(ne1, c[u]) ? kEmptySlot : v;
+            y[u] = v;
+          }
+          // one bucket load per entry; kEmptySlot is above smax, so a
+          // discarded entry never counts
+          uint32_t more = 0;
+#pragma unroll
+          for (int u = 0; u < kGroupUnroll; ++u) {
+            const Bucket4 v = tab[HashSlot(y[u], shift)];
+            const uint32_t d = min(min(v.s[0] ^ y[u], v.s[1] ^ y[u]), min(v.s[2] ^ y[u], v.s[3] ^ y[u]));
+            const bool in_range = y[u] >= smin && y[u] <= smax;
+            hits += (d == 0 && in_range) ? 1u : 0u;
+            more |= (d != 0 && in_range && v.s[3] != kEmptySlot) ? (1u << u) : 0u;
+          }
+          if (__ballot(more != 0)) {  // an overflowed bucket (rare): follow the chain
+#pragma unroll
+            for (int u = 0; u < kGroupUnroll; ++u)
+              if ((more >> u) & 1u) {
+                uint32_t b = HashSlot(y[u], shift);
+                bool go = true;
+                while (go) {
+                  b = (b + 1) & mask;
+                  const Bucket4 v = tab[b];
+                  const bool hit = (v.s[0] == y[u]) || (v.s[1] == y[u]) || (v.s[2] == y[u]) || (v.s[3] == y[u]);
+                  hits += hit ? 1u : 0u;
+                  go = !hit && v.s[3] != kEmptySlot;
+                }
+              }
+          }
+        }
+        if (a.walked) {
+          const u64 run_len = WaveSum(clen);  // every lane takes part
+          walked += lane == 0 ? run_len : 0;
+        }
+        local += hits;
+        if (a.bk.bounds) {
+          const uint64_t first = sub + f;
+          BucketAccAdd(a.bk, &bacc, BucketOf(a.bk, Gather(a.cs, a.bk.start_v, first)), hits);
+        }
+      }
+    }
+  }
+  if (a.bk.bounds) BucketFlush(a.bk, &bacc);
+  local = WaveSum(local);
+  lower = WaveSum(lower);
+  walked = WaveSum(walked);
+  if (lane == 0) {
+    if (local) {
+      atomicAdd(a.acc, local);
+      atomicAdd(a.acc_level, local);
+    }
+    if (a.walked && walked) atomicAdd(a.walked, walked);
+    if (a.lower && lower) atomicAdd(a.lower, lower);
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void UpperStartKernel(const uint64_t* rp, const uint32_t* cols,
+                                                             uint64_t n, uint64_t* up) {
+  for (uint64_t v = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; v < n;
+       v += GridStride())
+    up[v] = LowerBound(cols, rp[v], rp[v + 1], static_cast<uint32_t>(v + 1));
+}
+
 }  // namespace
 
 size_t ScanTempWords(uint64_t n) {
@@ -1152,6 +1392,19 @@ void LaunchCompact(const CompactArgs& a, hipStream_t stream) {
   if (a.out_vals && a.it.nchunks)
     hipLaunchKernelGGL(CompactValuesKernel, dim3(GridFor(a.it.nchunks, kWavesPerBlock)), dim3(kThreads), 0, stream, a);
   if (a.it.nseg) hipLaunchKernelGGL(SegmentRanksKernel, dim3(GridFor(a.it.nseg, kThreads)), dim3(kThreads), 0, stream, a);
+}
+
+void LaunchGroupCount(const GroupCountArgs& a, uint32_t max_grid, hipStream_t stream) {
+  if (a.npaths == 0) return;
+  const uint64_t units = (a.npaths + kGroupUnit - 1) / kGroupUnit;
+  hipLaunchKernelGGL(GroupCountKernel, dim3(GridFor(units, kWavesPerBlock, max_grid)), dim3(kThreads), 0,
+                     stream, a);
+}
+
+void LaunchUpperStart(const uint64_t* rp, const uint32_t* cols, uint64_t n, uint64_t* up,
+                      hipStream_t stream) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(UpperStartKernel, dim3(GridFor(n, kThreads)), dim3(kThreads), 0, stream, rp, cols, n, up);
 }
 
 void LaunchExpand(const ExpandArgs& a, hipStream_t stream) {

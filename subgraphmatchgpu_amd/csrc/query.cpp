@@ -343,6 +343,13 @@ void FillConds(Cond* dst, uint8_t* n, const CondList& c) {
   for (size_t i = 0; i < c.size(); ++i) dst[i] = c[i];
 }
 
+// Every pair of query vertices adjacent.
+bool IsClique(const Query& q) {
+  for (int u = 0; u < q.VertexCount(); ++u)
+    if (static_cast<int>(q.Connectivity()[u].size()) != q.VertexCount() - 1) return false;
+  return true;
+}
+
 }  // namespace
 
 bool MakeLightEnginePlan(int query_type, int compress_level, bool enable_ordering,
@@ -362,6 +369,7 @@ bool MakeLightEnginePlan(int query_type, int compress_level, bool enable_orderin
   out->nops = static_cast<int>(ops.size());
   out->direct_count = (compress_level == kClSpecial && CanDirectCount(query_type)) ? 1 : 0;
   out->reuse = 1;
+  out->clique = IsClique(q) ? 1 : 0;
   for (size_t i = 0; i < ops.size(); ++i) {
     OpDesc& d = out->ops[i];
     d.kind = ops[i].op;
@@ -406,6 +414,7 @@ bool MakeGpsmEnginePlan(int query_type, bool enable_ordering, bool reuse, Engine
   out->compress_level = -1;
   out->enable_ordering = enable_ordering ? 1 : 0;
   out->nv = nv;
+  out->clique = IsClique(q) ? 1 : 0;
   std::string text = "GPSM Q" + std::to_string(query_type) + " ord=" +
                      (enable_ordering ? "1" : "0") + "\n";
   int k = 0;

@@ -135,6 +135,12 @@ struct EnginePlan {
   int nops;
   int direct_count;    // COUNT uses the per-path closed form (LIGHTDirectCount)
   int reuse;           // extend the parent's candidate set where provably equal
+  // The pattern is a clique: every level's output (partial instances and
+  // candidates alike) counts the injective maps of a clique prefix that
+  // satisfy conditions symmetric under any vertex relabeling, so a run over
+  // all start vertices gives the same count and level outputs on any
+  // isomorphic copy of the graph.
+  int clique;
   OpDesc ops[kMaxOps];
 };
 

@@ -82,8 +82,11 @@ int main(int argc, char** argv) {
     return 1;
   }
   if (argc == 10) sme_set_hub_bitmaps(ctx, std::strtoull(argv[9], nullptr, 10), 1ull << 30);
-  // TUNE="lane_path_max,item_grid,batch_bytes[,super_chunks[,stream_ratio]]" sets the
-  // schedule tunables
+  // RANK=1 keeps a degree-ranked copy of the graph (clique plans over all
+  // start vertices run on it; the output then says ranked=1)
+  if (const char* env = std::getenv("RANK")) sme_set_rank_order(ctx, std::atoi(env));
+  // TUNE="lane_path_max,item_grid,batch_bytes[,super_chunks[,stream_ratio[,group_max]]]"
+  // sets the schedule tunables
   if (const char* env = std::getenv("TUNE")) {
     sme_tuning t{};
     char* s = nullptr;
@@ -92,6 +95,7 @@ int main(int argc, char** argv) {
     if (*s == ',') t.batch_bytes = std::strtoull(s + 1, &s, 10);
     if (*s == ',') t.super_chunks = static_cast<uint32_t>(std::strtoul(s + 1, &s, 10));
     if (*s == ',') t.stream_ratio = static_cast<uint32_t>(std::strtoul(s + 1, &s, 10));
+    if (*s == ',') t.group_max = static_cast<uint32_t>(std::strtoul(s + 1, &s, 10));
     sme_set_tuning(ctx, &t);
   }
   sme_plan* plan = nullptr;
@@ -132,10 +136,15 @@ int main(int argc, char** argv) {
     }
   }
   std::vector<uint64_t> bcounts(bounds.empty() ? 0 : bounds.size() - 1);
-  const int rc = bounds.empty()
-                     ? sme_count(ctx, g, plan, vb, static_cast<uint32_t>(ve), &count, levels, &st)
-                     : sme_count_buckets(ctx, g, plan, bounds.data(), bcounts.size(), bcounts.data(),
-                                         &count, levels, &st);
+  // RANKED=1: the range is in the degree-ranked copy's ids (sme_count_ranked);
+  // NOLEVELS=1: no level outputs are asked for
+  const bool ranked_ids = std::getenv("RANKED") && std::atoi(std::getenv("RANKED"));
+  uint64_t* lv = (std::getenv("NOLEVELS") && std::atoi(std::getenv("NOLEVELS"))) ? nullptr : levels;
+  const int rc = !bounds.empty()
+                     ? sme_count_buckets(ctx, g, plan, bounds.data(), bcounts.size(), bcounts.data(),
+                                         &count, lv, &st)
+                 : ranked_ids ? sme_count_ranked(ctx, g, plan, vb, static_cast<uint32_t>(ve), &count, lv, &st)
+                              : sme_count(ctx, g, plan, vb, static_cast<uint32_t>(ve), &count, lv, &st);
   if (rc != SME_OK) {
     std::fprintf(stderr, "count: rc=%d %s\n", rc, sme_last_error(ctx));
     return 1;
@@ -144,6 +153,7 @@ int main(int argc, char** argv) {
   for (int i = 0; i < sme_plan_levels(plan); ++i)
     std::printf(i ? ",%llu" : "%llu", static_cast<unsigned long long>(levels[i]));
   std::printf(" batches=%llu", static_cast<unsigned long long>(st.batches));
+  if (st.ranked) std::printf(" ranked=1");
   if (!bounds.empty()) {
     std::printf(" buckets=");
     for (size_t k = 0; k < bcounts.size(); ++k)

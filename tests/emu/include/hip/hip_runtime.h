@@ -121,6 +121,9 @@ inline void __builtin_amdgcn_wave_barrier() { (void)emu::WaveExchange(emu::kBall
 inline unsigned int __builtin_amdgcn_readfirstlane(unsigned int v) {
   return static_cast<unsigned int>(emu::WaveExchange(emu::kReadLane, v, 0));
 }
+inline int __builtin_amdgcn_readlane(int v, int lane) {
+  return static_cast<int>(emu::WaveExchange(emu::kReadLane, static_cast<unsigned>(v), static_cast<unsigned>(lane)));
+}
 template <typename T>
 inline T __shfl(T v, int src, int width = 64) {
   (void)width;
@@ -128,6 +131,12 @@ inline T __shfl(T v, int src, int width = 64) {
 }
 inline int __clzll(long long x) { return x ? __builtin_clzll(static_cast<unsigned long long>(x)) : 64; }
 inline unsigned int __popcll(unsigned long long x) { return static_cast<unsigned>(__builtin_popcountll(x)); }
+inline int __ffsll(long long x) { return __builtin_ffsll(x); }
+inline unsigned int min(unsigned int a, unsigned int b) { return a < b ? a : b; }
+// HIP declares __umul24 with an int result (the low 32 bits of the product)
+inline int __umul24(unsigned int a, unsigned int b) {
+  return static_cast<int>((a & 0xffffffu) * (b & 0xffffffu));
+}
 inline unsigned long long __lanemask_lt() {
   const unsigned lane = threadIdx.x & 63u;
   return lane ? (~0ull >> (64 - lane)) : 0ull;
@@ -136,6 +145,11 @@ inline void __syncthreads() { emu::BlockBarrier(); }
 inline unsigned int atomicOr(unsigned int* p, unsigned int v) {
   const unsigned int o = *p;
   *p = o | v;
+  return o;
+}
+inline unsigned int atomicCAS(unsigned int* p, unsigned int cmp, unsigned int v) {
+  const unsigned int o = *p;
+  if (o == cmp) *p = v;
   return o;
 }
 inline unsigned long long atomicAdd(unsigned long long* p, unsigned long long v) {

@@ -267,6 +267,44 @@ def test_emu_match_lists(emu, graphs):
     assert r.returncode == 3
 
 
+def test_emu_group_count_and_rank_order(emu, graphs):
+    """Group counting (segment staged in LDS as a bucket hash, lists
+    streamed in tiles of 64) alone and beside the item schedule (cap 8), on
+    the original ids and on the degree-ranked copy: counts and levels equal
+    the oracle; full-range clique runs (and count-only runs of any plan)
+    take the ranked copy, ranges do not; ranked tasks of a partition sum to
+    the full count."""
+    g, p = graphs["rmat7"]
+    base = dict(os.environ, ASAN_OPTIONS="detect_leaks=0")
+
+    def go(q, env, vb=0, ve=None):
+        r = subprocess.run([emu, p, "6", str(q), "3", "1", str(vb), str(g.n if ve is None else ve),
+                            str(1 << 30)], capture_output=True, text=True, env=dict(base, **env), timeout=600)
+        assert r.returncode == 0, r.stderr[-2000:]
+        parts = r.stdout.split()
+        lv = parts[1].split("=")[1]
+        return int(parts[0].split("=")[1]), [int(x) for x in lv.split(",")], "ranked=1" in r.stdout
+
+    for q in (0, 3, 4, 7):
+        want = oracle.light_count(g.row_ptrs, g.cols, q, 3)
+        for tune, rank in (("0,0,0,4,0,512", "0"), ("0,0,0,4,0,8", "1"), ("0,0,0,1,0,512", "1")):
+            cnt, lv, ranked = go(q, {"TUNE": tune, "RANK": rank})
+            assert (cnt, lv) == (want[0], want[1]), (q, tune, rank)
+            assert ranked == (rank == "1" and q in (0, 3, 7))
+        a, b = 17, 90
+        cnt, lv, ranked = go(q, {"TUNE": "0,0,0,4,0,512", "RANK": "1"}, a, b)
+        assert not ranked
+        assert (cnt, lv) == oracle.light_count(g.row_ptrs, g.cols, q, 3, v_begin=a, v_end=b)
+        cnt, _, ranked = go(q, {"TUNE": "0,0,0,4,0,512", "RANK": "1", "NOLEVELS": "1"})
+        assert ranked and cnt == want[0]
+        bounds = [0, 40, 41, 100, g.n]
+        parts = [go(q, {"TUNE": "0,0,0,4,0,512", "RANK": "1", "RANKED": "1"}, a, b)
+                 for a, b in zip(bounds[:-1], bounds[1:])]
+        assert sum(x[0] for x in parts) == want[0]
+        if q in (0, 3, 7):
+            assert [sum(x) for x in zip(*[x[1] for x in parts])] == want[1]
+
+
 def test_pool_stream_ordered_reuse(emu):
     """Shared pool: a block freed by one stream stays out of the free list
     until its free event completes, even after the freeing stream allocates
