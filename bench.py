@@ -85,6 +85,53 @@ def cpu_baseline(g, seconds_target=15.0, chunk=64, seed=12345):
     }
 
 
+REF_CPU = os.path.join(ROOT, "oracle", "_ref", "patternmatchcpu")
+
+
+def reference_cpu_config1(sme):
+    """The reference's own CPU path (src/cpu CPU_WCOJ, built from its sources
+    into oracle/_ref by `make -C oracle ref`) on configuration 1 (R-MAT
+    scale 14, edge factor 16, triangle) with every host thread: its printed
+    elapsed_time (enumeration only, src/cpu/CPUWCOJ.h:29-66), beside this
+    engine's time for the same count on the GPU."""
+    import re
+    import subprocess
+    import tempfile
+
+    if not os.path.exists(REF_CPU):
+        return {"error": "oracle/_ref/patternmatchcpu is not built"}
+    g = sme.gen_rmat(14, 16, 0.57, 0.19, 0.19, SEED)
+    threads = os.cpu_count() or 1
+    cpu_model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            cpu_model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "rmat14.bin")
+        sme.write_bin(path, g)
+        r = subprocess.run([REF_CPU, "-f", path, "-d", "0", "-a", "0", "-q", "0", "-t", str(threads)],
+                           capture_output=True, text=True, timeout=300)
+    m = re.search(r"total_match_count=(\d+), elapsed_time=([0-9.]+)ms", r.stdout)
+    if r.returncode != 0 or not m:
+        return {"error": f"reference run failed (rc {r.returncode}): {r.stdout[-200:]} {r.stderr[-200:]}"}
+    count, ms = int(m.group(1)), float(m.group(2))
+    dev = sme.Device(0, budget_bytes=4 << 30)
+    dg = dev.upload(g)
+    plan = sme.Plan(0)
+    dev.count(dg, plan)
+    t = time.perf_counter()
+    gpu = dev.count(dg, plan)
+    gpu_ms = 1e3 * (time.perf_counter() - t)
+    dev.close()
+    if gpu.count != count:
+        raise AssertionError(f"configuration 1: reference {count} vs engine {gpu.count}")
+    return {"value": count / (ms / 1e3), "unit": "embeddings/s", "cores": threads, "kind": "reference",
+            "cpu_model": cpu_model, "sample": "configuration 1 in full: R-MAT scale 14 edge factor 16, "
+            f"triangle, CPU_WCOJ -t {threads}", "count": count, "elapsed_ms": ms, "engine_gpu_ms": gpu_ms}
+
+
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "intersect_rmat20_traffic.json")
 # Revision of the intersect kernel's memory behaviour; a committed traffic
 # figure is used only when it was measured on the same revision.
@@ -138,6 +185,10 @@ def main():
     ap.add_argument("--plain-limit-s", type=float, default=2.0,
                     help="replay only this many seconds of the stream into the plain path "
                          "(it cannot keep up; its backlog drains after the cut)")
+    ap.add_argument("--no-sub", action="store_true",
+                    help="rmat20: skip the ER-batch and serving sub-benchmarks of the default line")
+    ap.add_argument("--sub-serve-seconds", type=float, default=5.0,
+                    help="stream length of the serving sub-benchmark (served and plain replay it all)")
     ap.add_argument("--er-graphs", type=int, default=20000)
     ap.add_argument("--per-graph-sample", type=int, default=1000,
                     help="members timed with one pass each (er-batch)")
@@ -163,10 +214,10 @@ def main():
     import subgraphmatchgpu_amd as sme
 
     if args.workload == "er-batch":
-        bench_er_batch(args, sme)
+        print(json.dumps(bench_er_batch(args, sme)))
         return
     if args.workload == "serve":
-        bench_serve(args, sme)
+        print(json.dumps(bench_serve(args, sme, plain_limit_s=args.plain_limit_s)))
         return
 
     t0 = time.perf_counter()
@@ -306,16 +357,42 @@ def main():
             out["roofline"]["traffic_gbs"] = tr["bytes_per_launch"] / 1e9 / (avg_ms / 1e3)
     except (OSError, ValueError, KeyError):
         pass
+    dev.close()
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline(g)
         except Exception as e:  # reported, never fatal to the GPU number
             out["cpu_baseline"] = {"error": str(e)}
-    dev.close()
+        out["cpu_baseline"]["reference"] = reference_cpu_config1(sme)
+    del g
+    if not args.no_sub:
+        # configuration 4 (many uneven graphs in one pass, autotuned) and
+        # configuration 5 (served stream vs the plain path on the same
+        # requests), bounded so that the whole run stays within minutes
+        er = bench_er_batch(args, sme)
+        out["er_batch"] = {
+            "value": er["value"], "unit": er["unit"], "ms_per_step": er["ms_per_step"],
+            "workload": er["config"]["workload"], "totals": er["config"]["totals"],
+            "speedup_vs_one_pass_per_member": er["config"]["speedup_vs_one_pass_per_member"],
+            "one_pass_per_member": er["config"]["one_pass_per_member"],
+            "packed_default_schedule": er["config"]["packed_default_schedule"],
+            "tuning": er["config"]["autotune"]["tuning"], "roofline": er["roofline"],
+        }
+        sv = bench_serve(args, sme, seconds=args.sub_serve_seconds, plain_limit_s=None)
+        c = sv["config"]
+        out["serve"] = {
+            "workload": c["workload"], "requests": c["served"]["requests"],
+            "p50_ms": c["served"]["p50_ms"], "p99_ms": c["served"]["p99_ms"],
+            "throughput_rps": c["served"]["throughput_rps"],
+            "plain": {k: c["plain"][k] for k in ("requests", "p50_ms", "p99_ms", "throughput_rps")},
+            "throughput_vs_plain": c["throughput_vs_plain"], "counts_checked": c["counts_checked"],
+            "passes": c["served"]["stats"]["passes"],
+        }
     print(json.dumps(out))
 
 
 def bench_er_batch(args, sme):
+    """Configuration 4 (see the module docstring); returns the JSON dict."""
     import numpy as np
 
     t0 = time.perf_counter()
@@ -414,7 +491,7 @@ def bench_er_batch(args, sme):
                      "avg_launch_ms": avg_ms, "launches": dom["launches"]},
     }
     dev.close()
-    print(json.dumps(out))
+    return out
 
 
 SERVE_QUERIES = (0, 1, 2, 3, 4, 7)
@@ -487,15 +564,19 @@ def replay(srv, stream, limit_s=None):
     return [results[k] for k in range(len(ids))], len(ids), wall
 
 
-def bench_serve(args, sme):
+def bench_serve(args, sme, seconds=None, plain_limit_s=None):
+    """Configuration 5 on a stream of `seconds` (default --serve-seconds);
+    the plain path replays the first plain_limit_s seconds of the same
+    stream (None: all of it).  Returns the JSON dict."""
     import numpy as np
 
+    seconds = args.serve_seconds if seconds is None else seconds
     t0 = time.perf_counter()
     g = sme.gen_rmat(args.serve_scale, args.serve_edge_factor, 0.57, 0.19, 0.19, SEED)
     log(f"graph rmat{args.serve_scale} ef{args.serve_edge_factor}: n={g.n} nnz={g.nnz} "
         f"({time.perf_counter() - t0:.1f}s)")
-    stream = request_stream(g.n, args.serve_seconds, args.serve_rate, seed=SEED)
-    log(f"stream: {len(stream)} requests over {args.serve_seconds:.0f}s")
+    stream = request_stream(g.n, seconds, args.serve_rate, seed=SEED)
+    log(f"stream: {len(stream)} requests over {seconds:.0f}s")
     cap = int(args.budget_gb * (1 << 30))
     hub = dict(hub_min_degree=args.hub_min_degree, hub_max_bytes=int(args.hub_max_gb * (1 << 30)))
 
@@ -516,14 +597,14 @@ def bench_serve(args, sme):
     # plain path: one request at a time in arrival order, whole budget each
     srv = sme.Server(g, mem_cap_bytes=cap, slots=1, merge=False, **hub)
     log("plain: replaying")
-    res, sent, wall = replay(srv, stream, args.plain_limit_s)
+    res, sent, wall = replay(srv, stream, plain_limit_s)
     st = srv.stats()
     srv.close()
     lat = np.array([r[1] for r in res])
     plain = {"requests": sent, "wall_s": wall, "throughput_rps": sent / wall,
              "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
              "mean_ms": float(lat.mean()), "stats": st,
-             "stream_cut_at_s": args.plain_limit_s if sent < len(stream) else None}
+             "stream_cut_at_s": plain_limit_s if sent < len(stream) else None}
     log(f"plain: {plain}")
     plain_counts = [r[0] for r in res]
     if plain_counts != served_counts[:len(plain_counts)]:
@@ -544,7 +625,7 @@ def bench_serve(args, sme):
         "data": "synthetic (seeded R-MAT + seeded Poisson request stream)",
         "config": {
             "workload": f"RMAT scale {args.serve_scale} edge factor {args.serve_edge_factor} (n={g.n}, "
-                        f"nnz={g.nnz}); Poisson {args.serve_rate:.0f} req/s for {args.serve_seconds:.0f}s, "
+                        f"nnz={g.nnz}); Poisson {args.serve_rate:.0f} req/s for {seconds:.0f}s, "
                         "patterns {Q0,Q1,Q2,Q3,Q4,Q7} weights 0.3/0.2/0.2/0.15/0.1/0.05, "
                         "start range 1-100% of vertices",
             "mem_cap_gb": args.budget_gb,
@@ -554,7 +635,7 @@ def bench_serve(args, sme):
             "counts_checked": len(plain_counts),
         },
     }
-    print(json.dumps(out))
+    return out
 
 
 if __name__ == "__main__":

@@ -1089,7 +1089,7 @@ void ScanImpl(const T* in, uint64_t n, uint64_t* out, uint64_t* tmp, hipStream_t
 //
 // One wave per run of paths sharing a segment S.  S is staged in LDS as a
 // sorted array (set[0, n)) and a hash table of 4-slot buckets at set +
-// kGroupMax (at most two values per bucket on average).  A value goes to the
+// kGroupMax (at most one value per bucket on average).  A value goes to the
 // first free slot of its bucket, or of the next bucket when all four are
 // taken; a lookup reads a whole bucket with one 16-byte LDS load and moves
 // on only when that bucket is full and lacks the value, so lanes rarely
@@ -1099,14 +1099,19 @@ void ScanImpl(const T* in, uint64_t n, uint64_t* out, uint64_t* tmp, hipStream_t
 constexpr uint32_t kEmptySlot = 0xffffffffu;   // never a vertex id (n < 2^32 - 1)
 constexpr uint64_t kGroupUnit = 256;           // paths per work unit of a wave
 constexpr int kGroupUnroll = 8;                // tiles of 64 list entries in flight per wave
-constexpr uint32_t kGroupLdsWords = 3 * kGroupMax;  // sorted values + buckets (load <= 1/2)
+constexpr uint32_t kGroupLdsWords = 5 * kGroupMax;  // sorted values + buckets (<= 1 value per bucket on average)
 
 struct alignas(16) Bucket4 {
   uint32_t s[4];
 };
 
+// Multiplicative hash on a 24-bit product (v_mul_u32_u24 is full rate, a
+// 32-bit multiply is not): the high byte of x is folded into the low bits,
+// then the top bits of the low 32 bits of the product with an odd 24-bit
+// constant select the bucket.  __umul24 returns int, so the product is
+// shifted as unsigned.
 __device__ __forceinline__ uint32_t HashSlot(uint32_t x, uint32_t shift) {
-  return (x * 2654435761u) >> shift;  // Knuth's multiplicative hash
+  return static_cast<uint32_t>(__umul24(x ^ (x >> 24), 0x9E3779u)) >> shift;
 }
 
 __device__ __forceinline__ void BucketInsert(uint32_t* tab, uint32_t shift, uint32_t mask, uint32_t x) {
@@ -1142,6 +1147,7 @@ __device__ __forceinline__ uint64_t ReadLane64(uint64_t v, uint32_t lane) {
 
 __device__ __forceinline__ uint32_t Scalar32(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
+template <bool kNe>
 __global__ __launch_bounds__(kThreads) void GroupCountKernel(GroupCountArgs a) {
   __shared__ alignas(16) uint32_t lds_set[kWavesPerBlock][kGroupLdsWords];
   const int lane = threadIdx.x & (kWave - 1);
@@ -1174,8 +1180,8 @@ __global__ __launch_bounds__(kThreads) void GroupCountKernel(GroupCountArgs a) {
         if (s != staged) {
           const uint32_t* src = a.seg.base + a.seg.beg[s];
           n = static_cast<uint32_t>(slen);
-          uint32_t bbits = 4;  // buckets: a power of two >= n / 2
-          while ((2u << bbits) < n) ++bbits;
+          uint32_t bbits = 3;  // buckets: a power of two >= n
+          while ((1u << bbits) < n) ++bbits;
           shift = 32 - bbits;
           mask = (1u << bbits) - 1;
           WaveSync();  // earlier readers of the stage are done
@@ -1224,7 +1230,9 @@ __global__ __launch_bounds__(kThreads) void GroupCountKernel(GroupCountArgs a) {
         }
         // tiles of 64 consecutive entries: the owner of tile t is the
         // number of children whose tiles all lie before t, found by one
-        // ballot, so per tile the owner's bounds are scalar
+        // ballot, so per tile the owner's bounds are scalar.  In the run's
+        // flat tile numbering, child c's entries are flat offsets
+        // [64 excl_c, lend_c) and entry f is cols[abase_c + f].
         const uint32_t own_tiles = (clen + kWave - 1) / kWave;
         uint32_t incl = own_tiles;
 #pragma unroll
@@ -1233,66 +1241,72 @@ __global__ __launch_bounds__(kThreads) void GroupCountKernel(GroupCountArgs a) {
           if (lane >= o) incl += t;
         }
         const uint32_t excl = incl - own_tiles;
+        const uint64_t abase = wb - static_cast<uint64_t>(excl) * kWave;  // wraps; only sums are used
+        const uint32_t lend = excl * kWave + clen;
         const uint32_t ntiles = ReadLane32(incl, kWave - 1);
-        uint32_t hits = 0;
+        u64 hits = 0;  // wave-uniform
         for (uint32_t tb = 0; tb < ntiles; tb += kGroupUnroll) {
-          uint32_t y[kGroupUnroll], c[kGroupUnroll];
-          bool ok[kGroupUnroll];
+          uint32_t y[kGroupUnroll];
+          u64 okm[kGroupUnroll];
 #pragma unroll
           for (int u = 0; u < kGroupUnroll; ++u) {  // issue every load first
-            const uint32_t tt = tb + u;
-            const uint32_t cu = static_cast<uint32_t>(__popcll(__ballot(incl <= tt)));
-            c[u] = cu < kWave - 1 ? cu : kWave - 1;
-            const uint32_t off = (tt - ReadLane32(excl, c[u])) * kWave + lane;
-            const uint32_t len = ReadLane32(clen, c[u]);
-            const uint64_t base = ReadLane64(wb, c[u]);
-            ok[u] = tt < ntiles && off < len;
-            // a lane past the window (or a tile past the last) reads a
-            // valid entry and is discarded below
-            y[u] = a.cols[base + (ok[u] ? off : 0u)];
+            // a tile past the last repeats the last one with no lane valid
+            const uint32_t tt = tb + u < ntiles ? tb + u : ntiles - 1;
+            const uint32_t c = static_cast<uint32_t>(__popcll(__ballot(incl <= tt)));
+            const uint32_t end = ReadLane32(lend, c);
+            const uint32_t off = tt * kWave + lane;
+            // a lane past its owner's window reads the owner's last entry
+            y[u] = a.cols[ReadLane64(abase, c) + (off < end ? off : end - 1)];
+            u64 m = tb + u < ntiles ? __ballot(off < end) : 0ull;
+            if (kNe) {
+              // the values tested here already passed the < / > window;
+              // entries equal to a != operand of their path drop out
+              const uint32_t x0 = ReadLane32(ne0, c), x1 = ReadLane32(ne1, c);
+              m &= __ballot(y[u] != x0 && y[u] != x1);
+            }
+            okm[u] = m;
           }
-#pragma unroll
-          for (int u = 0; u < kGroupUnroll; ++u) {
-            uint32_t v = ok[u] ? y[u] : kEmptySlot;
-            if (a.nne > 0) v = v == ReadLane32(ne0, c[u]) ? kEmptySlot : v;
-            if (a.nne > 1) v = v == ReadLane32(ne1, c[u]) ? kEmptySlot : v;
-            y[u] = v;
-          }
-          // one bucket load per entry; kEmptySlot is above smax, so a
-          // discarded entry never counts
+          // one bucket load per entry; a valid entry is a hit when its
+          // bucket holds it, and follows the chain when that bucket is full
+          // (per lane, one bit per tile)
           uint32_t more = 0;
 #pragma unroll
           for (int u = 0; u < kGroupUnroll; ++u) {
             const Bucket4 v = tab[HashSlot(y[u], shift)];
             const uint32_t d = min(min(v.s[0] ^ y[u], v.s[1] ^ y[u]), min(v.s[2] ^ y[u], v.s[3] ^ y[u]));
-            const bool in_range = y[u] >= smin && y[u] <= smax;
-            hits += (d == 0 && in_range) ? 1u : 0u;
-            more |= (d != 0 && in_range && v.s[3] != kEmptySlot) ? (1u << u) : 0u;
+            hits += __popcll(okm[u] & __ballot(d == 0));
+            const u64 chain = okm[u] & __ballot(d != 0 && v.s[3] != kEmptySlot);
+            more |= ((chain >> lane) & 1ull) ? (1u << u) : 0u;
           }
-          if (__ballot(more != 0)) {  // an overflowed bucket (rare): follow the chain
+          if (__ballot(more != 0)) {  // rare per entry: walk only the pending (lane, tile) pairs
+            uint32_t extra = 0;
+            while (more) {
+              const int u = __ffs(more) - 1;
+              more &= more - 1;
+              uint32_t yy = y[0];
 #pragma unroll
-            for (int u = 0; u < kGroupUnroll; ++u)
-              if ((more >> u) & 1u) {
-                uint32_t b = HashSlot(y[u], shift);
-                bool go = true;
-                while (go) {
-                  b = (b + 1) & mask;
-                  const Bucket4 v = tab[b];
-                  const bool hit = (v.s[0] == y[u]) || (v.s[1] == y[u]) || (v.s[2] == y[u]) || (v.s[3] == y[u]);
-                  hits += hit ? 1u : 0u;
-                  go = !hit && v.s[3] != kEmptySlot;
-                }
+              for (int k = 1; k < kGroupUnroll; ++k) yy = u == k ? y[k] : yy;
+              uint32_t b = HashSlot(yy, shift);
+              bool go = true;
+              while (go) {
+                b = (b + 1) & mask;
+                const Bucket4 v = tab[b];
+                const bool hit = (v.s[0] == yy) || (v.s[1] == yy) || (v.s[2] == yy) || (v.s[3] == yy);
+                extra += hit ? 1u : 0u;
+                go = !hit && v.s[3] != kEmptySlot;
               }
+            }
+            hits += WaveSum(extra);
           }
         }
         if (a.walked) {
           const u64 run_len = WaveSum(clen);  // every lane takes part
           walked += lane == 0 ? run_len : 0;
         }
-        local += hits;
+        local += lane == 0 ? hits : 0;
         if (a.bk.bounds) {
           const uint64_t first = sub + f;
-          BucketAccAdd(a.bk, &bacc, BucketOf(a.bk, Gather(a.cs, a.bk.start_v, first)), hits);
+          BucketAccAdd(a.bk, &bacc, BucketOf(a.bk, Gather(a.cs, a.bk.start_v, first)), lane == 0 ? hits : 0);
         }
       }
     }
@@ -1397,8 +1411,11 @@ void LaunchCompact(const CompactArgs& a, hipStream_t stream) {
 void LaunchGroupCount(const GroupCountArgs& a, uint32_t max_grid, hipStream_t stream) {
   if (a.npaths == 0) return;
   const uint64_t units = (a.npaths + kGroupUnit - 1) / kGroupUnit;
-  hipLaunchKernelGGL(GroupCountKernel, dim3(GridFor(units, kWavesPerBlock, max_grid)), dim3(kThreads), 0,
-                     stream, a);
+  const dim3 grid(GridFor(units, kWavesPerBlock, max_grid));
+  if (a.nne)
+    hipLaunchKernelGGL(GroupCountKernel<true>, grid, dim3(kThreads), 0, stream, a);
+  else
+    hipLaunchKernelGGL(GroupCountKernel<false>, grid, dim3(kThreads), 0, stream, a);
 }
 
 void LaunchUpperStart(const uint64_t* rp, const uint32_t* cols, uint64_t n, uint64_t* up,

@@ -132,6 +132,7 @@ inline T __shfl(T v, int src, int width = 64) {
 inline int __clzll(long long x) { return x ? __builtin_clzll(static_cast<unsigned long long>(x)) : 64; }
 inline unsigned int __popcll(unsigned long long x) { return static_cast<unsigned>(__builtin_popcountll(x)); }
 inline int __ffsll(long long x) { return __builtin_ffsll(x); }
+inline int __ffs(unsigned int x) { return __builtin_ffs(static_cast<int>(x)); }
 inline unsigned int min(unsigned int a, unsigned int b) { return a < b ? a : b; }
 // HIP declares __umul24 with an int result (the low 32 bits of the product)
 inline int __umul24(unsigned int a, unsigned int b) {
