@@ -4,7 +4,9 @@ Workload (one "step"): on a resident R-MAT scale-20 graph (edge factor 16,
 a/b/c/d = 0.57/0.19/0.19/0.05, seed 1, symmetrized, deduplicated), count the
 4-clique (Q3) and then the diamond (Q2) over every start vertex, each with
 symmetry breaking, through the C ABI (libsme.so).  value = embeddings counted
-per second of step wall time, inputs already in HBM.
+per second of step wall time, inputs already in HBM.  The graph keeps a
+degree-ranked copy (the 4-clique's count and levels are those of the
+original ids) and counting levels use the group schedule (group_max 512).
 
 Besides the JSON fields of the harness contract, the line carries
   roofline     -- the dominant kernel family ("intersect": the set
@@ -30,11 +32,20 @@ cap); latency p50/p99 and throughput are reported beside the plain
 one-request-at-a-time path replaying the first --plain-limit-s seconds of the
 same stream, and every request's count is checked against the plain path's.
 On R-MAT scale 22 with edge factor 32 a single house or 5-clique request
-over the highest-degree start vertices runs for minutes to hours (a 5-clique
-pass from start vertex 0 alone counts 1.04e12 instances in 45 s), so that
-stream cannot be replayed within a benchmark run.
+over the highest-degree start vertices runs for minutes to hours (requests
+are start ranges of the original ids, which the ranked copy cannot serve),
+so that stream cannot be replayed within a benchmark run.
 
-Usage: python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--workload rmat20|er-batch|serve]
+Fourth workload (--workload rmat22): R-MAT scale 22, edge factor 32, with
+5-clique and house under a 32 GB intermediate cap on the degree-ranked copy,
+as ranked start-range tasks for a bounded time per pattern; the rest of the
+range is projected from a sampled set of start vertices (bench_rmat22).
+
+The default line (rmat20) also carries, as sub-objects, a bounded run of the
+second workload ("er_batch") and of the third on a 5 s stream replayed in
+full by both the served and the plain path ("serve").
+
+Usage: python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--workload rmat20|er-batch|serve|rmat22]
 """
 import argparse
 import json
@@ -132,13 +143,13 @@ def reference_cpu_config1(sme):
             f"triangle, CPU_WCOJ -t {threads}", "count": count, "elapsed_ms": ms, "engine_gpu_ms": gpu_ms}
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "intersect_rmat20_traffic.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "intersect_group_rmat20_traffic.json")
 # Revision of the intersect kernel's memory behaviour; a committed traffic
 # figure is used only when it was measured on the same revision.
 KERNEL_REV = "ranked-copy-group-count-bucket-hash-tiles64"
 
 
-def summarize_pmc(fetch_csv, write_csv, calib_bytes, kernel="VerifyKernel"):
+def summarize_pmc(fetch_csv, write_csv, calib_bytes, kernel="GroupCountKernel"):
     """HBM bytes per dispatch of `kernel` from two rocprofv3 --pmc runs
     (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each scaled by what the
     same counter read for the calibration copy of calib_bytes (the largest
@@ -176,7 +187,11 @@ def main():
     ap.add_argument("--hub-min-degree", type=int, default=256,
                     help="adjacency bitmaps for vertices of at least this degree (0: off)")
     ap.add_argument("--hub-max-gb", type=float, default=8.0)
-    ap.add_argument("--workload", choices=("rmat20", "er-batch", "serve"), default="rmat20")
+    ap.add_argument("--workload", choices=("rmat20", "er-batch", "serve", "rmat22"), default="rmat20")
+    ap.add_argument("--r22-seconds", type=float, default=90.0,
+                    help="rmat22: enumeration seconds per pattern before the rest is projected")
+    ap.add_argument("--r22-sample-seconds", type=float, default=45.0,
+                    help="rmat22: seconds of sampled single-vertex counts for the projection")
     ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-slots", type=int, default=4)
@@ -200,12 +215,14 @@ def main():
                          "schedule (rmat20)")
     ap.add_argument("--no-rank-order", action="store_true",
                     help="rmat20: no degree-ranked copy (clique counts then walk the original ids)")
-    ap.add_argument("--summarize-pmc", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "CALIB_MIB"),
-                    help="print HBM traffic per intersect launch from two rocprofv3 --pmc runs")
+    ap.add_argument("--summarize-pmc", nargs="+", metavar="ARG",
+                    help="FETCH_CSV WRITE_CSV CALIB_MIB [KERNEL]: print HBM traffic per launch of KERNEL "
+                         "(default GroupCountKernel) from two rocprofv3 --pmc runs")
     args = ap.parse_args()
     if args.summarize_pmc:
-        f, w, mib = args.summarize_pmc
-        print(json.dumps(summarize_pmc(f, w, int(mib) << 20)))
+        f, w, mib = args.summarize_pmc[:3]
+        kernel = args.summarize_pmc[3] if len(args.summarize_pmc) > 3 else "GroupCountKernel"
+        print(json.dumps(summarize_pmc(f, w, int(mib) << 20, kernel)))
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU; --gpus must be 1"}))
@@ -218,6 +235,11 @@ def main():
         return
     if args.workload == "serve":
         print(json.dumps(bench_serve(args, sme, plain_limit_s=args.plain_limit_s)))
+        return
+    if args.workload == "rmat22":
+        if args.budget_gb == 64.0:
+            args.budget_gb = 32.0  # the configuration's intermediate cap
+        print(json.dumps(bench_rmat22(args, sme)))
         return
 
     t0 = time.perf_counter()
@@ -345,13 +367,14 @@ def main():
             },
         },
     }
-    # HBM bytes per launch measured by rocprofv3 counters for this same
-    # configuration (profiles/intersect_rmat20_traffic.json)
+    # HBM bytes per launch of the dominant kernel measured by rocprofv3
+    # counters for this same configuration (TRAFFIC_FILE)
     try:
         with open(TRAFFIC_FILE) as f:
             tr = json.load(f)
         if tr.get("config") == {"hub_min_degree": args.hub_min_degree, "schedule": dev.get_tuning(),
-                                "kernel_rev": KERNEL_REV}:
+                                "kernel_rev": KERNEL_REV, "rank_order": not args.no_rank_order} and \
+                tr.get("kernel", "") in out["roofline"]["kernel"]:
             out["roofline"]["traffic"] = tr["bytes_per_launch"]
             out["roofline"]["traffic_source"] = os.path.relpath(TRAFFIC_FILE, ROOT)
             out["roofline"]["traffic_gbs"] = tr["bytes_per_launch"] / 1e9 / (avg_ms / 1e3)
@@ -492,6 +515,133 @@ def bench_er_batch(args, sme):
     }
     dev.close()
     return out
+
+
+def bench_rmat22(args, sme):
+    """Configuration 3: R-MAT scale 22, edge factor 32, 5-clique and house,
+    intermediates capped at 32 GB (--budget-gb), on the degree-ranked copy.
+
+    Each pattern is counted as ranked start-range tasks (sme_count_ranked)
+    in increasing rank order, with task sizes adapted to take a few seconds,
+    until --r22-seconds of enumeration.  When time runs out, the rest of the
+    ranked range is estimated from a seeded stratified sample of single
+    start vertices (--r22-sample-seconds): the projected time is the
+    measured time plus, per stratum, its size times the mean sampled time
+    per vertex.  Reported per pattern: instances counted, ranked vertices
+    covered, batches, peak pool bytes, the dominant kernel's GB/s, and the
+    projection when the count is partial."""
+    import numpy as np
+
+    t0 = time.perf_counter()
+    g = sme.gen_rmat(22, 32, 0.57, 0.19, 0.19, SEED)
+    log(f"graph rmat22 ef32: n={g.n} nnz={g.nnz} ({time.perf_counter() - t0:.1f}s)")
+    budget = int(args.budget_gb * (1 << 30))
+    dev = sme.Device(0, budget_bytes=budget)
+    dev.set_rank_order(True)
+    t0 = time.perf_counter()
+    dg = dev.upload(g)
+    log(f"upload + ranked copy: {time.perf_counter() - t0:.1f}s")
+    dev.set_tuning(super_chunks=4, group_max=512)
+    rng = np.random.default_rng(SEED)
+    # work proxy of a ranked start vertex: its squared degree (ranked ids are
+    # in increasing degree order); base cuts split the proxy into 2,048 parts
+    deg = np.sort(np.diff(g.row_ptrs.astype(np.int64))).astype(np.float64)
+    cw = np.concatenate([[0.0], np.cumsum(deg * deg)])
+    base = np.unique(np.concatenate([np.searchsorted(cw, np.linspace(0, cw[-1], 2049)[1:]), [g.n]]))
+    base = base[base > 0]
+    results = {}
+    total_counted, total_s = 0, 0.0
+    for q, name in ((7, "5-clique"), (4, "house")):
+        plan = sme.Plan(q)
+        clique = q == 7
+        dev.profile_reset()
+        dev.profile(True)
+        count, lv, batches, peak, spent, r = 0, None, 0, 0, 0.0, 0
+        rate = None  # seconds per unit of squared degree, from the last task
+        while r < g.n and spent < args.r22_seconds:
+            # next task: the ranked vertices whose squared degrees sum to
+            # about 3 s at the last task's rate (at least one base cut)
+            if rate:
+                want = cw[r] + 3.0 / rate
+                b = int(np.searchsorted(cw, want, side="right")) - 1
+            else:
+                b = int(base[np.searchsorted(base, r, side="right")])
+            b = max(b, int(base[np.searchsorted(base, r, side="right")]))
+            b = min(g.n, max(b, r + 1))
+            res = dev.count_ranked(dg, plan, r, b, levels=clique)
+            count += res.count
+            batches += res.batches
+            peak = max(peak, res.peak_pool_bytes)
+            if clique:
+                lv = res.levels if lv is None else [x + y for x, y in zip(lv, res.levels)]
+            dt = res.elapsed_ms / 1e3
+            spent += dt
+            log(f"  {name}: ranked [{r}, {b}) {res.count} in {dt:.2f}s, {res.batches} batches "
+                f"(total {count} in {spent:.1f}s)")
+            w = cw[b] - cw[r]
+            rate = dt / w if w > 0 and dt > 0.05 else rate
+            r = b
+        dev.profile(False)
+        prof = dev.profile_read()
+        fam = max(prof, key=lambda k: prof[k]["ms"])
+        dom = prof[fam]
+        gbs = (dom["bytes"] / 1e9) / (dom["ms"] / 1e3) if dom["ms"] else 0.0
+        ent = {"query": f"Q{q} ({name})", "complete": r >= g.n, "count": count,
+               "ranked_vertices_counted": r, "seconds": spent, "batches": batches,
+               "peak_pool_gib": peak / 2**30, "rate_per_s": count / spent if spent else 0.0,
+               "dominant_kernel": {"family": fam, "ms": dom["ms"], "GB": dom["bytes"] / 1e9,
+                                   "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS}}
+        if clique:
+            ent["levels"] = lv
+        if r < g.n:
+            # stratified sample of single start vertices over [r, n)
+            strata = np.unique(np.linspace(r, g.n, 9).astype(np.int64))
+            per, est, used = [], 0.0, 0.0
+            for lo, hi in zip(strata[:-1], strata[1:]):
+                if used > args.r22_sample_seconds:
+                    break
+                vs = np.sort(rng.choice(np.arange(lo, hi), size=min(4, hi - lo), replace=False))
+                ts = []
+                for v in vs:
+                    res = dev.count_ranked(dg, plan, int(v), int(v) + 1, levels=False)
+                    ts.append(res.elapsed_ms / 1e3)
+                    used += ts[-1]
+                    log(f"  {name}: sample ranked vertex {v}: {res.count} in {ts[-1]:.2f}s")
+                per.append({"range": [int(lo), int(hi)], "sampled": len(ts), "mean_s": float(np.mean(ts))})
+                est += (hi - lo) * float(np.mean(ts))
+            sampled_all = len(per) == len(strata) - 1
+            ent["projection"] = {
+                "method": "remaining ranked range in 8 equal strata, 4 seeded random start vertices "
+                          "each counted alone; remaining time = sum of stratum size x mean time",
+                "strata": per, "complete_sample": sampled_all,
+                "remaining_seconds": est if sampled_all else None,
+                "projected_total_seconds": spent + est if sampled_all else None,
+            }
+        results[name] = ent
+        total_counted += count
+        total_s += spent
+    dev.close()
+    return {
+        "metric": "embeddings/sec (1 GPU, 32 GB intermediate cap)",
+        "value": total_counted / total_s if total_s else 0.0,
+        "unit": "embeddings/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 0,
+        "ms_per_step": 1e3 * total_s,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32/u64 integer",
+        "data": "synthetic (seeded R-MAT)",
+        "config": {
+            "workload": f"RMAT scale 22 edge factor 32 (n={g.n}, nnz={g.nnz}); 5-clique and house, LIGHT, "
+                        "degree-ranked copy, ranked start-range tasks",
+            "budget_gb": args.budget_gb,
+            "seconds_per_pattern": args.r22_seconds,
+            "patterns": results,
+        },
+    }
 
 
 SERVE_QUERIES = (0, 1, 2, 3, 4, 7)
