@@ -1149,7 +1149,7 @@ __device__ __forceinline__ uint32_t Scalar32(uint32_t v) { return __builtin_amdg
 
 template <bool kNe>
 __global__ __launch_bounds__(kThreads) void GroupCountKernel(GroupCountArgs a) {
-  __shared__ alignas(16) uint32_t lds_set[kWavesPerBlock][kGroupLdsWords];
+  __shared__ uint32_t lds_set[kWavesPerBlock][kGroupLdsWords] __attribute__((aligned(16)));
   const int lane = threadIdx.x & (kWave - 1);
   uint32_t* set = lds_set[threadIdx.x / kWave];
   const Bucket4* tab = reinterpret_cast<const Bucket4*>(set + kGroupMax);
