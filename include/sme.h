@@ -141,6 +141,22 @@ void sme_plan_destroy(sme_plan* p);
 int sme_count(sme_ctx* ctx, const sme_graph* g, const sme_plan* p, uint32_t v_begin,
               uint32_t v_end, uint64_t* out_count, uint64_t* level_counts, sme_stats* stats);
 
+/* sme_count that allocates nothing during the call.  No counterpart in the
+ * reference; it is for callers that cannot allocate device memory while
+ * they count (code under stream capture, tight serving loops).  Every
+ * intermediate comes from a device array of sme_static_scratch_bytes()
+ * (64 MiB) that the library holds on each device it runs on until the
+ * process ends (set up at sme_open), instead of the context's pool; no
+ * hipMalloc, hipFree or HIP event is created by the call, and the batches
+ * shrink to fit the array.  A call whose single unsplittable batch needs
+ * more than the array is rejected with SME_ERR_OOM and a message naming the
+ * limit.  All calls on one device share the array: calls that overlap on
+ * one device race.  Results equal sme_count's. */
+int sme_count_static(sme_ctx* ctx, const sme_graph* g, const sme_plan* p, uint32_t v_begin,
+                     uint32_t v_end, uint64_t* out_count, uint64_t* level_counts,
+                     sme_stats* stats);
+uint64_t sme_static_scratch_bytes(void);
+
 /* sme_count over a start range [r_begin, r_end) of the graph's degree-ranked
  * copy (sme_set_rank_order), in that copy's ids: the tasks of any partition
  * of [0, n) sum to the full count (and, for cliques, to the full level

@@ -136,6 +136,9 @@ def library() -> ctypes.CDLL:
                               ctypes.c_int),
         "sme_graph_hubs": ([vp, u64p], ctypes.c_int),
         "sme_set_rank_order": ([vp, ctypes.c_int], ctypes.c_int),
+        "sme_count_static": ([vp, vp, vp, ctypes.c_uint32, ctypes.c_uint32, u64p, u64p,
+                              ctypes.POINTER(_Stats)], ctypes.c_int),
+        "sme_static_scratch_bytes": ([], ctypes.c_uint64),
         "sme_count_ranked": ([vp, vp, vp, ctypes.c_uint32, ctypes.c_uint32, u64p, u64p,
                               ctypes.POINTER(_Stats)], ctypes.c_int),
         "sme_graph_ranked": ([vp, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
@@ -333,6 +336,21 @@ class Device:
         st = _Stats()
         _check(lib.sme_count(self._h, dg._h, plan._h, v_begin, v_end, ctypes.byref(cnt), lv,
                              ctypes.byref(st)), self._h)
+        return CountResult(int(cnt.value), [int(lv[i]) for i in range(plan.levels)], int(st.batches),
+                           int(st.d2h_syncs), int(st.peak_pool_bytes), float(st.elapsed_ms),
+                           bool(st.ranked))
+
+    def count_static(self, dg: "DeviceGraph", plan: Plan, v_begin: int = 0, v_end: int | None = None) -> CountResult:
+        """sme_count through the no-allocation entry point (intermediates in
+        the device's static scratch array)."""
+        lib = library()
+        if v_end is None:
+            v_end = dg.n
+        cnt = ctypes.c_uint64()
+        lv = (ctypes.c_uint64 * MAX_LEVELS)()
+        st = _Stats()
+        _check(lib.sme_count_static(self._h, dg._h, plan._h, v_begin, v_end, ctypes.byref(cnt), lv,
+                                    ctypes.byref(st)), self._h)
         return CountResult(int(cnt.value), [int(lv[i]) for i in range(plan.levels)], int(st.batches),
                            int(st.d2h_syncs), int(st.peak_pool_bytes), float(st.elapsed_ms),
                            bool(st.ranked))

@@ -219,6 +219,28 @@ int sme_count(sme_ctx* ctx, const sme_graph* g, const sme_plan* p, uint32_t v_be
   });
 }
 
+int sme_count_static(sme_ctx* ctx, const sme_graph* g, const sme_plan* p, uint32_t v_begin,
+                     uint32_t v_end, uint64_t* out_count, uint64_t* level_counts, sme_stats* stats) {
+  return Guarded(ctx, [&]() -> int {
+    if (!ctx || !g || !p || !out_count) return Report(ctx, SME_ERR_INVALID, "null argument");
+    std::string err;
+    sme::RunStats rs;
+    const sme::Status s = sme::RunPlanNoAlloc(&ctx->impl, &g->impl, &p->plan, v_begin, v_end, out_count,
+                                              level_counts, &rs, &err);
+    if (s != sme::kOk) return Report(ctx, s, err);
+    if (stats) {
+      stats->batches = rs.batches;
+      stats->d2h_syncs = rs.d2h_syncs;
+      stats->peak_pool_bytes = rs.peak_pool_bytes;
+      stats->elapsed_ms = rs.elapsed_ms;
+      stats->ranked = rs.ranked;
+    }
+    return SME_OK;
+  });
+}
+
+uint64_t sme_static_scratch_bytes(void) { return sme::StaticScratchBytes(); }
+
 int sme_count_ranked(sme_ctx* ctx, const sme_graph* g, const sme_plan* p, uint32_t r_begin,
                      uint32_t r_end, uint64_t* out_count, uint64_t* level_counts, sme_stats* stats) {
   return Guarded(ctx, [&]() -> int {

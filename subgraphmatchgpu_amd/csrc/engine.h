@@ -66,6 +66,13 @@ class Context {
   int device() const { return device_; }
   hipStream_t stream() const { return stream_; }
   DevicePool& pool() { return *pool_; }
+  // Runs of this context allocate from `p` from now on; returns the pool
+  // used before.
+  DevicePool* SwapPool(DevicePool* p) {
+    DevicePool* old = pool_;
+    pool_ = p;
+    return old;
+  }
 
   // Allocation accounting of this context's runs.  With `limit` set (a
   // reservation in a shared pool) the batch planner sees at most
@@ -178,6 +185,18 @@ Status RunPlan(Context* ctx, const DevGraph* g, const EnginePlan* plan, uint32_t
 Status RunPlanBuckets(Context* ctx, const DevGraph* g, const EnginePlan* plan,
                       const uint32_t* bounds, uint64_t nbuckets, uint64_t* counts, uint64_t* count,
                       uint64_t* level_counts, RunStats* stats, std::string* err);
+
+// The no-allocation variant of RunPlan: every intermediate comes from the
+// device's static scratch array (static_scratch.hip) instead of the
+// context's pool, and no HIP events are created.  Calls that overlap on
+// one device share that array and race.
+Status RunPlanNoAlloc(Context* ctx, const DevGraph* g, const EnginePlan* plan, uint32_t v_begin,
+                      uint32_t v_end, uint64_t* count, uint64_t* level_counts, RunStats* stats,
+                      std::string* err);
+// Once per device, with the device current: locates the static scratch
+// array (hipGetSymbolAddress) and sets up its pool.  Called by Context::Open.
+Status InitStaticScratch(int device, std::string* err);
+uint64_t StaticScratchBytes();
 
 // Match list of a full-materialization plan (GPSM / GPSM_REUSE) over start
 // vertices [v_begin, v_end): up to max_rows tuples of plan->nv u32 written

@@ -77,6 +77,10 @@ Status Context::OpenShared(int device, DevicePool* shared, std::string* err) {
     *err = "hipHostMalloc failed";
     return kErrHip;
   }
+  // the no-allocation entry point's scratch; a failure is reported by that
+  // entry point, not here
+  std::string scratch_err;
+  (void)InitStaticScratch(device, &scratch_err);
   return kOk;
 }
 
@@ -339,7 +343,8 @@ class Runner {
     const int nacc = 1 + 4 * plan_->nops;
     Buf<unsigned long long> acc(pool_, nacc);
     acc_ = acc.get();
-    SME_HIP(hipMemsetAsync(acc_, 0, sizeof(unsigned long long) * nacc, st_));
+    // a fill kernel, not hipMemsetAsync: callers may capture the stream
+    LaunchFill64(reinterpret_cast<uint64_t*>(acc_), nacc, 0, st_);
     std::memset(lane_lists_, 0, sizeof(lane_lists_));
     std::memset(host_level_, 0, sizeof(host_level_));
     host_count_ = 0;

@@ -25,13 +25,27 @@ bool DevicePool::Init(size_t bytes) {
   return true;
 }
 
+void DevicePool::InitExternal(void* base, size_t bytes) {
+  Release();
+  std::lock_guard<std::mutex> lk(mu_);
+  bytes = (bytes / kAlign) * kAlign;
+  if (!base || bytes == 0) return;
+  base_ = static_cast<char*>(base);
+  external_ = true;
+  capacity_ = bytes;
+  used_ = 0;
+  high_water_ = 0;
+  InsertFree(0, bytes);
+}
+
 void DevicePool::Release() {
   std::lock_guard<std::mutex> lk(mu_);
   ReclaimLocked(nullptr, true);
   for (hipEvent_t e : spare_events_) (void)hipEventDestroy(e);
   spare_events_.clear();
-  if (base_) (void)hipFree(base_);
+  if (base_ && !external_) (void)hipFree(base_);
   base_ = nullptr;
+  external_ = false;
   capacity_ = used_ = high_water_ = 0;
   free_by_off_.clear();
   free_by_size_.clear();

@@ -41,6 +41,9 @@ class DevicePool {
 
   // Reserves `bytes` of device memory.  Returns false if hipMalloc fails.
   bool Init(size_t bytes);
+  // Carves `bytes` of device memory the caller owns (no hipMalloc, and
+  // Release does not free it).
+  void InitExternal(void* base, size_t bytes);
   void Release();
 
   // nullptr when the request does not fit; `bytes` == 0 returns nullptr.
@@ -80,6 +83,7 @@ class DevicePool {
   std::vector<hipEvent_t> spare_events_;
 
   char* base_ = nullptr;
+  bool external_ = false;
   size_t capacity_ = 0;
   size_t used_ = 0;
   size_t high_water_ = 0;

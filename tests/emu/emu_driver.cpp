@@ -139,11 +139,14 @@ int main(int argc, char** argv) {
   // RANKED=1: the range is in the degree-ranked copy's ids (sme_count_ranked);
   // NOLEVELS=1: no level outputs are asked for
   const bool ranked_ids = std::getenv("RANKED") && std::atoi(std::getenv("RANKED"));
+  // STATIC=1: the no-allocation entry point (sme_count_static)
+  const bool no_alloc = std::getenv("STATIC") && std::atoi(std::getenv("STATIC"));
   uint64_t* lv = (std::getenv("NOLEVELS") && std::atoi(std::getenv("NOLEVELS"))) ? nullptr : levels;
   const int rc = !bounds.empty()
                      ? sme_count_buckets(ctx, g, plan, bounds.data(), bcounts.size(), bcounts.data(),
                                          &count, lv, &st)
                  : ranked_ids ? sme_count_ranked(ctx, g, plan, vb, static_cast<uint32_t>(ve), &count, lv, &st)
+                 : no_alloc   ? sme_count_static(ctx, g, plan, vb, static_cast<uint32_t>(ve), &count, lv, &st)
                               : sme_count(ctx, g, plan, vb, static_cast<uint32_t>(ve), &count, lv, &st);
   if (rc != SME_OK) {
     std::fprintf(stderr, "count: rc=%d %s\n", rc, sme_last_error(ctx));

@@ -67,6 +67,12 @@ hipError_t hipEventElapsedTime(float* ms, hipEvent_t a, hipEvent_t b);
 #define __forceinline__ inline
 #define __launch_bounds__(...)
 #define __shared__ static
+#define __align__(n) alignas(n)
+#define HIP_SYMBOL(x) (&(x))
+inline hipError_t hipGetSymbolAddress(void** dev, const void* sym) {
+  *dev = const_cast<void*>(sym);
+  return hipSuccess;
+}
 
 namespace emu {
 // Test hook: while set, hipEventQuery reports recorded events as not yet

@@ -18,6 +18,10 @@ bool DevicePool::Init(size_t bytes) {
   return true;
 }
 
+// The test build keeps one malloc per block here too (the product pool
+// carves the external array; this one only enforces its capacity).
+void DevicePool::InitExternal(void*, size_t bytes) { Init(bytes); }
+
 void DevicePool::Release() {
   std::lock_guard<std::mutex> lk(mu_);
   ReclaimLocked(nullptr, true);
