@@ -222,6 +222,12 @@ int sme_plan_vertices(const sme_plan* p);
  *                  one wavefront that keeps the segment in LDS and reads
  *                  each path's list in order, 64 consecutive entries at a
  *                  time (0: off, the default).
+ *   wg_path_min    counting levels: a path with at least this many pivot
+ *                  candidates is counted by a whole workgroup (256 threads
+ *                  that stage a window of another list in LDS, up to 8,192
+ *                  entries) instead of the 64-candidates-per-wavefront
+ *                  schedule; the reference gives every path one warp
+ *                  (GPSMBinSearch.cuh:36-63).  0: off (the default).
  * Settings apply to every later call on the context; results never depend
  * on them. */
 typedef struct {
@@ -231,6 +237,7 @@ typedef struct {
   uint64_t batch_bytes;
   uint32_t stream_ratio;
   uint32_t group_max;
+  uint64_t wg_path_min;
 } sme_tuning;
 int sme_set_tuning(sme_ctx* ctx, const sme_tuning* t);
 int sme_get_tuning(const sme_ctx* ctx, sme_tuning* t);
@@ -242,8 +249,10 @@ typedef struct {
 
 /* Autotuner: times the plans on a sample of the start vertices for each
  * candidate setting (coordinate descent over lane_path_max, super_chunks,
- * stream_ratio, item_grid and batch_bytes, starting from the context's current setting), installs the
- * fastest on the context and returns it.  With bounds (nbuckets >= 1) the
+ * item_grid, batch_bytes, stream_ratio, group_max and wg_path_min, starting
+ * from the context's current setting; every setting's sample runs three
+ * times and its median counts; a setting replaces the current one only when
+ * at least 2 % faster), installs the fastest on the context and returns it.  With bounds (nbuckets >= 1) the
  * sample is 8 evenly spaced blocks of consecutive buckets covering about
  * sample_fraction of the buckets, run through sme_count_buckets; without
  * (bounds == NULL) it is 8 evenly spaced start-vertex ranges covering

@@ -305,6 +305,7 @@ int sme_set_tuning(sme_ctx* ctx, const sme_tuning* t) {
   ctx->impl.tuning.batch_bytes = t->batch_bytes;
   ctx->impl.tuning.stream_ratio = t->stream_ratio;
   ctx->impl.tuning.group_max = t->group_max;
+  ctx->impl.tuning.wg_path_min = t->wg_path_min;
   return SME_OK;
 }
 
@@ -317,6 +318,7 @@ int sme_get_tuning(const sme_ctx* ctx, sme_tuning* t) {
   t->batch_bytes = ctx->impl.tuning.batch_bytes;
   t->stream_ratio = ctx->impl.tuning.stream_ratio;
   t->group_max = ctx->impl.tuning.group_max;
+  t->wg_path_min = ctx->impl.tuning.wg_path_min;
   return SME_OK;
 }
 
@@ -378,12 +380,13 @@ int sme_autotune(sme_ctx* ctx, const sme_graph* g, const sme_plan* const* plans,
     };
     auto to_c = [](const sme::Context::Tuning& t) {
       return sme_tuning{t.lane_path_max, t.item_grid, t.super_chunks, t.batch_bytes,
-                        t.stream_ratio, t.group_max};
+                        t.stream_ratio, t.group_max, t.wg_path_min};
     };
     auto same = [](const sme::Context::Tuning& a, const sme::Context::Tuning& b) {
       return a.lane_path_max == b.lane_path_max && a.item_grid == b.item_grid &&
              a.super_chunks == b.super_chunks && a.batch_bytes == b.batch_bytes &&
-             a.stream_ratio == b.stream_ratio && a.group_max == b.group_max;
+             a.stream_ratio == b.stream_ratio && a.group_max == b.group_max &&
+             a.wg_path_min == b.wg_path_min;
     };
     const sme::Context::Tuning start = c.tuning;
     int n = 0;
@@ -418,8 +421,9 @@ int sme_autotune(sme_ctx* ctx, const sme_graph* g, const sme_plan* const* plans,
     const uint64_t batch_opts[] = {0, 4ull << 30, 1ull << 30, 256ull << 20, 64ull << 20};
     const uint32_t stream_opts[] = {0, 4, 8, 16, 32};
     const uint32_t group_opts[] = {0, 128, 512};
-    const int nopts[] = {5, 5, 5, 5, 5, 3};
-    for (int dim = 0; dim < 6; ++dim) {
+    const uint64_t wg_opts[] = {0, 2048, 8192};
+    const int nopts[] = {5, 5, 5, 5, 5, 3, 3};
+    for (int dim = 0; dim < 7; ++dim) {
       sme::Context::Tuning dim_best = cur;
       double dim_ms = cur_ms;
       for (int k = 0; k < nopts[dim]; ++k) {
@@ -430,6 +434,7 @@ int sme_autotune(sme_ctx* ctx, const sme_graph* g, const sme_plan* const* plans,
         if (dim == 3) t.batch_bytes = batch_opts[k];
         if (dim == 4) t.stream_ratio = stream_opts[k];
         if (dim == 5) t.group_max = group_opts[k];
+        if (dim == 6) t.wg_path_min = wg_opts[k];
         if (same(t, cur)) continue;
         double t_ms = 0;
         rc = measure(t, &t_ms);
@@ -475,6 +480,7 @@ int sme_server_open(int device, const uint64_t* row_ptrs, const uint32_t* cols, 
     c.tuning.batch_bytes = cfg->tuning.batch_bytes;
     c.tuning.stream_ratio = cfg->tuning.stream_ratio;
     c.tuning.group_max = cfg->tuning.group_max;
+    c.tuning.wg_path_min = cfg->tuning.wg_path_min;
     std::string err;
     const sme::Status st = s->impl.Open(device, row_ptrs, cols, n, nnz, c, &err);
     if (st != sme::kOk) {

@@ -107,6 +107,9 @@ class Context {
   //                  share of the pool)
   //   stream_ratio   pivot rule for a reused segment plus one adjacency
   //                  list (see PrepComputeArgs; 0: shortest list)
+  //   wg_path_min    counting levels: a path with at least this many pivot
+  //                  items is counted by a whole workgroup (PathWorkgroupKernel)
+  //                  instead of the 64-items-per-wave schedule (0: off)
   //   group_max      counting levels that extend a reused segment with one
   //                  adjacency list: paths sharing a segment of at most this
   //                  many values are counted together by one wave that keeps
@@ -119,6 +122,7 @@ class Context {
     uint64_t batch_bytes = 0;
     uint32_t stream_ratio = 0;
     uint32_t group_max = 0;
+    uint64_t wg_path_min = 0;
   } tuning;
 
   // profiling

@@ -339,17 +339,19 @@ class Runner {
   void Run(uint32_t v_begin, uint32_t v_end, uint64_t* count, uint64_t* levels, RunStats* stats) {
     Analyze();
     // [total, per-level counts, per-level items counted by lanes, per-level
-    // values streamed by group counting, per-level group walk lower bound]
-    const int nacc = 1 + 4 * plan_->nops;
+    // values streamed by group counting, per-level group walk lower bound,
+    // per-level items walked by workgroup-per-path counting]
+    const int nacc = 1 + 5 * plan_->nops;
     Buf<unsigned long long> acc(pool_, nacc);
     acc_ = acc.get();
     // a fill kernel, not hipMemsetAsync: callers may capture the stream
     LaunchFill64(reinterpret_cast<uint64_t*>(acc_), nacc, 0, st_);
     std::memset(lane_lists_, 0, sizeof(lane_lists_));
+    std::memset(wg_lists_, 0, sizeof(wg_lists_));
     std::memset(host_level_, 0, sizeof(host_level_));
     host_count_ = 0;
     if (v_end > v_begin) Level0(v_begin, v_end);
-    unsigned long long dev[1 + 4 * kMaxOps];
+    unsigned long long dev[1 + 5 * kMaxOps];
     SME_HIP(hipMemcpyAsync(ctx_->pinned(), acc_, sizeof(unsigned long long) * nacc,
                            hipMemcpyDeviceToHost, st_));
     SME_HIP(hipStreamSynchronize(st_));
@@ -369,6 +371,14 @@ class Runner {
       for (int L = 0; L < plan_->nops; ++L) {
         gs.items += dev[1 + 2 * plan_->nops + L];
         gs.bytes += 8.0 * static_cast<double>(dev[1 + 3 * plan_->nops + L]);
+      }
+      // workgroup-per-path counting: 4 bytes per list per pivot item, as the
+      // item schedule of the same level
+      SlotStats& is = ctx_->slots[kSlotIntersect];
+      for (int L = 0; L < plan_->nops; ++L) {
+        const uint64_t items = dev[1 + 4 * plan_->nops + L];
+        is.items += items;
+        is.bytes += 4.0 * wg_lists_[L] * static_cast<double>(items);
       }
     }
     uint64_t total = host_count_ + dev[0];
@@ -791,6 +801,17 @@ class Runner {
       Timed t(ctx_, st_, kSlotGroup, P * (4.0 * (1 + ga.nlt + ga.ngt + nne) + 32.0 + 4.0), 0);
       LaunchGroupCount(ga, ctx_->tuning.item_grid, st_);
     }
+    // long paths of a counting level: one workgroup per path
+    const bool wg = op.kind == kComputeCount && !zero_copy && ctx_->tuning.wg_path_min > 0 &&
+                    nlists - 1 <= 2 && nne <= 2;
+    Buf<uint64_t> wg_list(pool_, wg ? 2 * P : 0);
+    Buf<unsigned long long> wg_count(pool_, wg ? 1 : 0);
+    if (wg) {
+      LaunchFill64(reinterpret_cast<uint64_t*>(wg_count.get()), 1, 0, st_);
+      pa.wg_min = ctx_->tuning.wg_path_min;
+      pa.wg_list = wg_list.get();
+      pa.wg_count = wg_count.get();
+    }
     // short paths of a counting level: one lane per path in the set-up kernel
     const bool lane_count =
         op.kind == kComputeCount && !zero_copy && !group && ctx_->tuning.lane_path_max > 0;
@@ -825,8 +846,34 @@ class Runner {
       LaunchPrepCompute(pa, st_);
     }
     Scan(size.get(), P, offs.get());
+    if (wg) {
+      ItemArgs it = BaseItems(f, 0, P, 0, nullptr, nullptr, 0);
+      it.src = g_->cols;
+      it.pstart = pstart.get();
+      it.piv = nlists > 1 ? piv.get() : nullptr;
+      it.nb = nb;
+      std::memcpy(it.bconn, lists, kQ);
+      if (reuse) {
+        it.has_reuse = 1;
+        it.reuse = f.reuse;
+      }
+      it.nne = nne;
+      std::memcpy(it.ne_v, ne_v, sizeof(it.ne_v));
+      if (pair) {
+        it.pair = f.cand[op.cun[0]];
+        it.pair_op = op.pair_op;
+        SetPairExclusion(f, op.cun[0], 0, &it);
+      }
+      it.acc = acc_;
+      it.acc_level = acc_ + 1 + L;
+      it.bk = bk_;
+      wg_lists_[L] = nlists;
+      Timed t(ctx_, st_, kSlotIntersect, 0, 0);
+      LaunchPathWorkgroup(it, pair ? kModeSumPair : kModeSum, wg_list.get(), wg_count.get(),
+                          acc_ + 1 + 4 * plan_->nops + L, ctx_->tuning.item_grid, st_);
+    }
     const uint64_t W = ReadU64(offs.get() + P);
-    if (lane_count && W == 0) return;
+    if ((lane_count || wg || group) && W == 0) return;
 
     if (zero_copy) {
       // the candidate set is a window of one adjacency list: no copy
@@ -1280,6 +1327,7 @@ class Runner {
   uint32_t* list_out_ = nullptr;
   uint64_t list_cap_ = 0, list_rows_ = 0;
   int lane_lists_[kMaxOps];  // lists per item of lane-counted levels
+  int wg_lists_[kMaxOps];    // lists per item of workgroup-counted levels
   uint64_t host_level_[kMaxOps];
   uint64_t levels_[kMaxOps];
   uint64_t host_count_ = 0;

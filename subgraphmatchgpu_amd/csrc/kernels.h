@@ -68,7 +68,15 @@ struct PrepComputeArgs {
   // paths whose reused segment holds at most group_cap values are counted
   // by GroupCountKernel: they get no items here (0: none)
   uint32_t group_cap;
+  // paths with at least wg_min pivot items (0: none) get no items either:
+  // (path, items) is appended to wg_list and PathWorkgroupKernel counts each
+  // with a whole workgroup; wg_count receives the number appended
+  uint64_t wg_min;
+  uint64_t* wg_list;
+  unsigned long long* wg_count;
 };
+
+constexpr uint32_t kWgStage = 8192;  // LDS entries a workgroup stages for one path
 
 constexpr uint32_t kStageEntries = 1024;  // LDS entries per wave for a staged list
 
@@ -283,6 +291,13 @@ void LaunchChunkPartition(const uint64_t* offs, uint64_t nseg, uint64_t nchunks,
 void LaunchVerify(const ItemArgs& a, int mode, uint32_t max_grid, hipStream_t stream);
 void LaunchCompact(const CompactArgs& a, hipStream_t stream);
 void LaunchGroupCount(const GroupCountArgs& a, uint32_t max_grid, hipStream_t stream);
+// Counting level (mode kModeSum or kModeSumPair) of the paths listed by the
+// set-up kernel (wg_list / wg_count, read on the device): one workgroup per
+// path stages a window of one list in LDS and walks the pivot with 256
+// threads.  items (nullable) receives the pivot items walked.
+void LaunchPathWorkgroup(const ItemArgs& a, int mode, const uint64_t* wg_list,
+                         const unsigned long long* wg_count, unsigned long long* items,
+                         uint32_t max_grid, hipStream_t stream);
 // up[v] = first index of rp[v]..rp[v+1] holding a neighbour > v.
 void LaunchUpperStart(const uint64_t* rp, const uint32_t* cols, uint64_t n, uint64_t* up,
                       hipStream_t stream);

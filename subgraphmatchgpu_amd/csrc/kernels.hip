@@ -253,7 +253,14 @@ __global__ __launch_bounds__(kThreads) void PrepComputeKernel(PrepComputeArgs a)
     uint64_t b, e;
     uint8_t pv;
     PrepPath(a, p, &b, &e, &pv);
-    a.size[p] = e - b;
+    uint64_t n = e - b;
+    if (a.wg_min && n >= a.wg_min) {  // counted by a workgroup of its own
+      const uint64_t k = atomicAdd(a.wg_count, 1ull);
+      a.wg_list[2 * k] = p;
+      a.wg_list[2 * k + 1] = n;
+      n = 0;
+    }
+    a.size[p] = n;
     a.pstart[p] = b;
     if (a.piv) a.piv[p] = pv;
   }
@@ -819,6 +826,11 @@ __global__ __launch_bounds__(kThreads) void PrepCountKernel(PrepCountArgs a) {
         if (n) val = LanePathCount(a, p, pv, b, e);
         nlane += n;
         n = 0;
+      } else if (a.pa.wg_min && n >= a.pa.wg_min) {  // counted by a workgroup of its own
+        const uint64_t k = atomicAdd(a.pa.wg_count, 1ull);
+        a.pa.wg_list[2 * k] = p;
+        a.pa.wg_list[2 * k + 1] = n;
+        n = 0;
       }
       a.pa.size[p] = n;
       a.pa.pstart[p] = b;
@@ -1082,6 +1094,87 @@ void ScanImpl(const T* in, uint64_t n, uint64_t* out, uint64_t* tmp, hipStream_t
   ScanImpl<unsigned long long>(sums, tiles, reinterpret_cast<uint64_t*>(sums),
                                tmp + tiles + 1, stream);
   hipLaunchKernelGGL(ScanTileApplyKernel<T>, dim3(tiles), dim3(kThreads), 0, stream, in, n, sums, o);
+}
+
+// --------------------------------------------------------------------------
+// PathWorkgroupKernel: one workgroup per long path of a counting level (the
+// paths the set-up kernel listed).  Per path, every list is cut to the value
+// window of the path's pivot items, the first non-hub window of at most
+// kWgStage entries is staged in LDS by the whole workgroup, and the 256
+// threads walk the pivot items in order (coalesced), testing each against
+// the staged window (LDS bisection), hub rows (one bit) or the remaining
+// window (global bisection).  At most two lists besides the pivot and two
+// != conditions (the caller checks).
+template <int kMode>
+__global__ __launch_bounds__(kThreads) void PathWorkgroupKernel(ItemArgs a, const uint64_t* wg_list,
+                                                                const unsigned long long* wg_count,
+                                                                unsigned long long* items) {
+  __shared__ uint32_t stage[kWgStage];
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t np = *wg_count;
+  u64 local = 0, walked = 0;
+  BucketAcc bacc{kNoBucket, 0};
+  for (uint64_t i = blockIdx.x; i < np; i += gridDim.x) {
+    const uint64_t p = wg_list[2 * i], n = wg_list[2 * i + 1];
+    const uint8_t pv = (a.nb + a.has_reuse > 1) ? a.piv[p] : 0;
+    const uint32_t* src = (a.has_reuse && pv == a.nb) ? a.reuse.base : a.src;
+    const uint64_t b = a.pstart[p];
+    const int nl = OtherLists(a);
+    ListRef l0{nullptr, nullptr, 0, 0}, l1{nullptr, nullptr, 0, 0};
+    CollectLists(a, p, pv, &l0, &l1);
+    const uint32_t x_first = src[b], x_last = src[b + n - 1];
+    if (nl > 0 && !l0.hrow) NarrowToWindow(&l0, x_first, x_last);
+    if (nl > 1 && !l1.hrow) NarrowToWindow(&l1, x_first, x_last);
+    int staged = -1;
+    if (nl > 0 && !l0.hrow && l0.hi - l0.lo <= kWgStage) staged = 0;
+    else if (nl > 1 && !l1.hrow && l1.hi - l1.lo <= kWgStage) staged = 1;
+    uint32_t nstage = 0;
+    __syncthreads();  // the previous path's readers of the stage are done
+    if (staged >= 0) {
+      const ListRef& r = staged == 0 ? l0 : l1;
+      nstage = static_cast<uint32_t>(r.hi - r.lo);
+      for (uint32_t k = threadIdx.x; k < nstage; k += kThreads) stage[k] = r.a[r.lo + k];
+    }
+    __syncthreads();
+    const uint32_t ne0 = a.nne > 0 ? Gather(a.cs, a.ne_v[0], p) : 0u;
+    const uint32_t ne1 = a.nne > 1 ? Gather(a.cs, a.ne_v[1], p) : 0u;
+    const uint32_t* parr = nullptr;
+    uint64_t plen = 0;
+    uint8_t pin = 0;
+    uint32_t pe0 = 0, pe1 = 0;
+    if (kMode == kModeSumPair) {
+      const uint64_t ps = SegIndex(a.pair, p);
+      parr = a.pair.base + a.pair.beg[ps];
+      plen = a.pair.len[ps];
+      if (a.pair_nexcl) {
+        pin = a.pair_excl_in[p];
+        pe0 = Gather(a.cs, a.pair_excl_v[0], p);
+        if (a.pair_nexcl > 1) pe1 = Gather(a.cs, a.pair_excl_v[1], p);
+      }
+    }
+    u64 v = 0;
+    for (uint64_t k = threadIdx.x; k < n; k += kThreads) {
+      const uint32_t x = src[b + k];
+      bool ok = true;
+      if (nl > 0) ok = Member(l0, staged == 0, stage, nstage, x);
+      if (nl > 1 && ok) ok = Member(l1, staged == 1, stage, nstage, x);
+      if (a.nne > 0) ok = ok && x != ne0;
+      if (a.nne > 1) ok = ok && x != ne1;
+      if (ok) v += kMode == kModeSum ? 1ull : PairCount(parr, plen, x, a.pair_op, a.pair_nexcl, pin, pe0, pe1);
+    }
+    local += v;
+    if (threadIdx.x == 0) walked += n;
+    if (a.bk.bounds) BucketAccAdd(a.bk, &bacc, BucketOf(a.bk, Gather(a.cs, a.bk.start_v, p)), v);
+  }
+  if (a.bk.bounds) BucketFlush(a.bk, &bacc);
+  local = WaveSum(local);
+  if (lane == 0) {
+    if (local) {
+      atomicAdd(a.acc, local);
+      atomicAdd(a.acc_level, local);
+    }
+    if (items && walked) atomicAdd(items, walked);
+  }
 }
 
 // --------------------------------------------------------------------------
@@ -1416,6 +1509,19 @@ void LaunchGroupCount(const GroupCountArgs& a, uint32_t max_grid, hipStream_t st
     hipLaunchKernelGGL(GroupCountKernel<true>, grid, dim3(kThreads), 0, stream, a);
   else
     hipLaunchKernelGGL(GroupCountKernel<false>, grid, dim3(kThreads), 0, stream, a);
+}
+
+void LaunchPathWorkgroup(const ItemArgs& a, int mode, const uint64_t* wg_list,
+                         const unsigned long long* wg_count, unsigned long long* items,
+                         uint32_t max_grid, hipStream_t stream) {
+  // one workgroup per listed path; blocks beyond the list exit at once
+  const dim3 grid(GridFor(static_cast<uint64_t>(max_grid ? max_grid : 2048), 1, max_grid));
+  if (mode == kModeSumPair)
+    hipLaunchKernelGGL(PathWorkgroupKernel<kModeSumPair>, grid, dim3(kThreads), 0, stream, a, wg_list,
+                       wg_count, items);
+  else
+    hipLaunchKernelGGL(PathWorkgroupKernel<kModeSum>, grid, dim3(kThreads), 0, stream, a, wg_list,
+                       wg_count, items);
 }
 
 void LaunchUpperStart(const uint64_t* rp, const uint32_t* cols, uint64_t n, uint64_t* up,

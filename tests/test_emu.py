@@ -166,7 +166,7 @@ def test_emu_er_batch_members(emu, tmp_path):
 
 
 @pytest.mark.parametrize("tune", ["4,0,0", "64,3,0", "1000000,1,20000", "0,8,0,2", "16,16,0,4",
-                                  "0,0,0,4,8", "8,0,0,2,1000"])
+                                  "0,0,0,4,8", "8,0,0,2,1000", "0,0,0,4,0,0,4", "8,2,0,4,0,8,16"])
 def test_emu_tunings_do_not_change_results(emu, graphs, tune):
     """Lane-per-path counting, tiny grids and tiny batch caps: same counts,
     levels and per-bucket counts."""

@@ -202,7 +202,9 @@ def test_self_loops_handled(dev):
 @pytest.mark.parametrize("tune", [dict(lane_path_max=8), dict(lane_path_max=1 << 20, item_grid=64),
                                   dict(item_grid=1, batch_bytes=1 << 16), dict(super_chunks=2),
                                   dict(super_chunks=16, lane_path_max=8, item_grid=512),
-                                  dict(stream_ratio=16), dict(stream_ratio=1000, lane_path_max=32)])
+                                  dict(stream_ratio=16), dict(stream_ratio=1000, lane_path_max=32),
+                                  dict(wg_path_min=8), dict(wg_path_min=64, lane_path_max=16, group_max=8),
+                                  dict(wg_path_min=2, item_grid=3)])
 def test_tunings_do_not_change_results(fixture_graphs, tune):
     g = fixture_graphs["rmat10"]
     d = sme.Device(0, budget_bytes=2 << 30)
