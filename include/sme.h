@@ -228,6 +228,12 @@ int sme_plan_vertices(const sme_plan* p);
  *                  entries) instead of the 64-candidates-per-wavefront
  *                  schedule; the reference gives every path one warp
  *                  (GPSMBinSearch.cuh:36-63).  0: off (the default).
+ *   clique_level_engine  0 (the default): a LIGHT clique plan with ordering
+ *                  (triangle, 4-, 5-, 6-clique) counts the cliques of each
+ *                  start vertex v inside its upper neighbourhood N+(v) by
+ *                  AND + popcount over bit rows of that neighbourhood; 1:
+ *                  through the level engine like every other plan.  Counts
+ *                  and level outputs are identical.
  * Settings apply to every later call on the context; results never depend
  * on them. */
 typedef struct {
@@ -238,6 +244,7 @@ typedef struct {
   uint32_t stream_ratio;
   uint32_t group_max;
   uint64_t wg_path_min;
+  uint32_t clique_level_engine;
 } sme_tuning;
 int sme_set_tuning(sme_ctx* ctx, const sme_tuning* t);
 int sme_get_tuning(const sme_ctx* ctx, sme_tuning* t);

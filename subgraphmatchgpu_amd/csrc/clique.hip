@@ -1,0 +1,400 @@
+// clique.hip -- k-clique counting over bitmaps of upper neighbourhoods
+// (CliqueArgs in kernels.h).
+//
+// Start vertices with at most 64 upper neighbours are counted by
+// CliqueSmallKernel: one wave per vertex, lane i holding row i in a register,
+// so every set below a row is one u64 and every step of the count is a
+// readlane, an AND and a popcount.  Larger neighbourhoods are counted in
+// batches: CliqueRowsKernel writes each vertex's rows to device memory (one
+// wave per row, its lanes testing 64 candidates at a time by binary search
+// in the row vertex's own upper list), then CliqueCountKernel takes one row
+// i per wave, keeps the sets it derives from row i in LDS and counts the
+// last level by AND + popcount over whole words, 64 clique members per word.
+//
+// The reference counts the same cliques by intersecting sorted adjacency
+// lists level by level (src/gpu/impl/light/op/LIGHTCompute.cuh,
+// LIGHTComputeCount.cuh:13-119); here the lists of one start vertex's
+// neighbourhood become bit rows, and one word operation replaces up to 64
+// membership probes.
+#include "kernels.h"
+
+namespace sme {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kWave = 64;
+constexpr int kWaves = kThreads / kWave;
+constexpr uint32_t kWordsMax = kLocalMax / 64;  // words of the longest row
+constexpr uint64_t kMaxGrid = 16384;
+
+using u64 = unsigned long long;
+
+uint64_t Grid(uint64_t n, uint64_t per_block) {
+  uint64_t g = (n + per_block - 1) / per_block;
+  if (g < 1) g = 1;
+  return g > kMaxGrid ? kMaxGrid : g;
+}
+
+// Orders the LDS accesses of a wave: every lane's earlier accesses complete
+// before any lane's later ones.
+__device__ __forceinline__ void Sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ u64 Sum64(u64 v) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+__device__ __forceinline__ uint32_t Lane32(uint32_t v, uint32_t lane) {
+  return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), static_cast<int>(lane)));
+}
+
+__device__ __forceinline__ u64 Lane64(u64 v, uint32_t lane) {
+  return (static_cast<u64>(Lane32(static_cast<uint32_t>(v >> 32), lane)) << 32) |
+         Lane32(static_cast<uint32_t>(v), lane);
+}
+
+__device__ __forceinline__ u64 First64(u64 v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v));
+  const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v >> 32));
+  return (static_cast<u64>(hi) << 32) | lo;
+}
+
+// Does the sorted a[lo, hi) hold key?
+__device__ __forceinline__ bool Holds(const uint32_t* a, uint64_t lo, uint64_t hi, uint32_t key) {
+  const uint64_t end = hi;
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) >> 1;
+    if (a[mid] < key)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo < end && a[lo] == key;
+}
+
+// Largest bucket index b with bounds[b] <= v.
+__device__ __forceinline__ uint64_t BucketIndex(const BucketRef& b, uint32_t v) {
+  if (b.table) return b.table[v - b.base];
+  uint64_t lo = 0, hi = b.nbuckets - 1;
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi + 1) >> 1;
+    if (b.bounds[mid] <= v)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ void AddBucket(const BucketRef& b, uint32_t v, u64 per_lane, int lane) {
+  const u64 tot = Sum64(per_lane);
+  if (lane == 0 && tot) atomicAdd(b.counts + BucketIndex(b, v), tot);
+}
+
+__device__ __forceinline__ void Flush(unsigned long long* acc, int j, u64 v, int lane) {
+  v = Sum64(v);
+  if (lane == 0 && v) atomicAdd(acc + j, v);
+}
+
+// ---------------------------------------------------------------------------
+// One wave per start vertex with d <= 64.  Lane i builds row i: for each i
+// the wave searches u_j (lane j > i) in N+(u_i) and the ballot is row i.
+template <int K>
+__global__ __launch_bounds__(kThreads) void CliqueSmallKernel(CliqueArgs a) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t wave = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) / kWave;
+  const uint64_t nwaves = static_cast<uint64_t>(gridDim.x) * blockDim.x / kWave;
+  const uint64_t nv = a.v_end - a.v_begin;
+  u64 c1 = 0, c2 = 0, c3 = 0, c4 = 0, c5 = 0, c6 = 0;
+  for (uint64_t t = wave; t < nv; t += nwaves) {
+    const uint32_t v = a.v_begin + static_cast<uint32_t>(t);
+    const uint64_t nb = a.up[v];
+    const uint64_t d = a.rp[v + 1] - nb;
+    if (d > kLocalSmall) {
+      if (lane == 0) {
+        if (d <= kLocalMax) {
+          const u64 pos = atomicAdd(a.nbig, 1ull);
+          a.big[pos] = v;
+          a.big_d[pos] = static_cast<uint32_t>(d);
+          c1 += 1;
+          c2 += d;
+        } else {
+          a.huge[atomicAdd(a.nhuge, 1ull)] = v;
+        }
+      }
+      continue;
+    }
+    if (lane == 0) {
+      c1 += 1;
+      c2 += d;
+    }
+    if (d < 2) continue;
+    const uint32_t du = static_cast<uint32_t>(d);
+    const uint32_t uj = static_cast<uint32_t>(lane) < du ? a.cols[nb + lane] : 0u;
+    u64 row = 0;  // lane i: row i
+    for (uint32_t i = 0; i + 1 < du; ++i) {
+      const uint32_t ui = Lane32(uj, i);
+      const uint64_t lb = a.up[ui], le = a.rp[ui + 1];
+      const bool ok = static_cast<uint32_t>(lane) > i && static_cast<uint32_t>(lane) < du &&
+                      Holds(a.cols, lb, le, uj);
+      const u64 r = __ballot(ok);
+      if (static_cast<uint32_t>(lane) == i) row = r;
+    }
+    const u64 tri = __popcll(row);
+    c3 += tri;
+    u64 vk = K == 3 ? tri : 0;  // this lane's share of the k-cliques from v
+    if (K >= 4) {
+      for (uint32_t i = 0; i + 2 < du; ++i) {
+        const u64 s1 = Lane64(row, i);
+        if (!s1) continue;
+        const u64 x4 = ((s1 >> lane) & 1) ? __popcll(s1 & row) : 0;
+        c4 += x4;
+        if (K == 4) {
+          vk += x4;
+          continue;
+        }
+        for (u64 m1 = s1; m1; m1 &= m1 - 1) {
+          const u64 s2 = s1 & Lane64(row, static_cast<uint32_t>(__builtin_ctzll(m1)));
+          if (!s2) continue;
+          const u64 x5 = ((s2 >> lane) & 1) ? __popcll(s2 & row) : 0;
+          c5 += x5;
+          if (K == 5) {
+            vk += x5;
+            continue;
+          }
+          for (u64 m2 = s2; m2; m2 &= m2 - 1) {
+            const u64 s3 = s2 & Lane64(row, static_cast<uint32_t>(__builtin_ctzll(m2)));
+            if (!s3) continue;
+            const u64 x6 = ((s3 >> lane) & 1) ? __popcll(s3 & row) : 0;
+            c6 += x6;
+            vk += x6;
+          }
+        }
+      }
+    }
+    if (a.bk.bounds) AddBucket(a.bk, v, vk, lane);
+  }
+  Flush(a.acc, 1, c1, lane);
+  Flush(a.acc, 2, c2, lane);
+  Flush(a.acc, 3, c3, lane);
+  if (K >= 4) Flush(a.acc, 4, c4, lane);
+  if (K >= 5) Flush(a.acc, 5, c5, lane);
+  if (K >= 6) Flush(a.acc, 6, c6, lane);
+}
+
+// ---------------------------------------------------------------------------
+// Big vertices: row items.  Vertex x of the batch owns items
+// [row_first[x], row_first[x+1]) - row_first[0]; the wave-uniform search
+// finds it.
+__device__ __forceinline__ uint64_t ItemVertex(const CliqueArgs& a, uint64_t t) {
+  const uint64_t base = a.row_first[0];
+  uint64_t lo = 0, hi = a.nverts - 1;
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi + 1) >> 1;
+    if (a.row_first[mid] - base <= t)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+// One wave per row: word c of row i is the ballot of "u_j in N+(u_i)" over
+// j = 64c + lane (j > i).  Words are gathered one per lane and stored 64 at
+// a time.
+__global__ __launch_bounds__(kThreads) void CliqueRowsKernel(CliqueArgs a) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t wave = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) / kWave;
+  const uint64_t nwaves = static_cast<uint64_t>(gridDim.x) * blockDim.x / kWave;
+  for (uint64_t t = wave; t < a.nitems; t += nwaves) {
+    const uint64_t x = ItemVertex(a, t);
+    const uint32_t v = a.verts[x];
+    const uint32_t d = static_cast<uint32_t>(a.row_first[x + 1] - a.row_first[x]);
+    const uint32_t i = static_cast<uint32_t>(t - (a.row_first[x] - a.row_first[0]));
+    const uint32_t W = (d + 63) / 64;
+    u64* out = a.rows + (a.word_first[x] - a.word_first[0]) + static_cast<uint64_t>(i) * W;
+    const uint64_t nb = a.up[v];
+    const uint32_t ui = a.cols[nb + i];
+    const uint64_t lb = a.up[ui], le = a.rp[ui + 1];
+    u64 keep = 0;
+    for (uint32_t c = 0; c < W; ++c) {
+      const uint32_t j = c * 64 + static_cast<uint32_t>(lane);
+      const bool ok = j > i && j < d && Holds(a.cols, lb, le, a.cols[nb + j]);
+      const u64 w = __ballot(ok);
+      if ((c & 63u) == static_cast<uint32_t>(lane)) keep = w;
+      if ((c & 63u) == 63u || c + 1 == W) {
+        if (static_cast<uint32_t>(lane) <= (c & 63u)) out[(c & ~63u) + lane] = keep;
+      }
+    }
+  }
+}
+
+// |s & row_j| over the words of row j that can be non-zero (bits above j).
+__device__ __forceinline__ u64 Probe(const u64* s, uint32_t W, const u64* rows, uint32_t j) {
+  const u64* r = rows + static_cast<uint64_t>(j) * W;
+  u64 x = 0;
+  for (uint32_t w = (j + 1) >> 6; w < W; ++w) x += __popcll(s[w] & r[w]);
+  return x;
+}
+
+// Sum over the members j of s (words [lo, W), in LDS) of |s & row_j|, as
+// per-lane partial sums.  Members are queued in LDS (q: 2 x 64 entries) so
+// every probe round has 64 of them, whatever the density of s.
+__device__ u64 LastLevel(const u64* s, uint32_t lo, uint32_t W, const u64* rows, uint32_t* q, int lane) {
+  const u64 below = lane ? (~0ull >> (64 - lane)) : 0ull;
+  u64 sum = 0;
+  uint32_t qn = 0;
+  for (uint32_t c = lo; c < W; ++c) {
+    const u64 bits = First64(s[c]);
+    if (!bits) continue;
+    if ((bits >> lane) & 1) q[qn + __popcll(bits & below)] = c * 64 + static_cast<uint32_t>(lane);
+    qn += __popcll(bits);
+    if (qn >= static_cast<uint32_t>(kWave)) {
+      Sync();
+      sum += Probe(s, W, rows, q[lane]);
+      const uint32_t rest = qn - kWave;
+      const uint32_t carry = static_cast<uint32_t>(lane) < rest ? q[kWave + lane] : 0u;
+      Sync();
+      if (static_cast<uint32_t>(lane) < rest) q[lane] = carry;
+      qn = rest;
+      Sync();
+    }
+  }
+  if (qn) {
+    Sync();
+    if (static_cast<uint32_t>(lane) < qn) sum += Probe(s, W, rows, q[lane]);
+  }
+  Sync();
+  return sum;
+}
+
+// One wave per row i of a big vertex: s1 = row i, and for K >= 5 every
+// s2 = s1 & row j (j in s1), s3 = s2 & row l, each kept in LDS while the
+// members below it are walked; the last set's members are probed by
+// LastLevel.
+template <int K>
+__global__ __launch_bounds__(kThreads) void CliqueCountKernel(CliqueArgs a) {
+  constexpr int kBufs = K >= 4 ? K - 3 : 1;
+  __shared__ u64 lds_s[kWaves][kBufs][kWordsMax];
+  __shared__ uint32_t lds_q[kWaves][2 * kWave];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wid = threadIdx.x / kWave;
+  u64* s1 = lds_s[wid][0];
+  u64* s2 = lds_s[wid][kBufs > 1 ? 1 : 0];
+  u64* s3 = lds_s[wid][kBufs > 2 ? 2 : 0];
+  uint32_t* q = lds_q[wid];
+  const uint64_t wave = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) / kWave;
+  const uint64_t nwaves = static_cast<uint64_t>(gridDim.x) * blockDim.x / kWave;
+  u64 c3 = 0, c4 = 0, c5 = 0, c6 = 0;
+  for (uint64_t t = wave; t < a.nitems; t += nwaves) {
+    const uint64_t x = ItemVertex(a, t);
+    const uint32_t d = static_cast<uint32_t>(a.row_first[x + 1] - a.row_first[x]);
+    const uint32_t i = static_cast<uint32_t>(t - (a.row_first[x] - a.row_first[0]));
+    const uint32_t W = (d + 63) / 64;
+    const u64* rows = a.rows + (a.word_first[x] - a.word_first[0]);
+    const u64* ri = rows + static_cast<uint64_t>(i) * W;
+    const uint32_t lo1 = (i + 1) >> 6;
+    u64 vk = 0;
+    Sync();
+    for (uint32_t w = lo1 + lane; w < W; w += kWave) {
+      const u64 xw = ri[w];
+      if (K >= 4) s1[w] = xw;
+      c3 += __popcll(xw);
+      if (K == 3) vk += __popcll(xw);
+    }
+    if (K >= 4) {
+      Sync();
+      if (K == 4) {
+        const u64 y = LastLevel(s1, lo1, W, rows, q, lane);
+        c4 += y;
+        vk += y;
+      } else {
+        for (uint32_t c = lo1; c < W; ++c) {
+          for (u64 m1 = First64(s1[c]); m1; m1 &= m1 - 1) {
+            const uint32_t j = c * 64 + static_cast<uint32_t>(__builtin_ctzll(m1));
+            const u64* rj = rows + static_cast<uint64_t>(j) * W;
+            const uint32_t lo2 = (j + 1) >> 6;
+            u64 nz = 0;
+            Sync();
+            for (uint32_t w = lo2 + lane; w < W; w += kWave) {
+              const u64 xw = s1[w] & rj[w];
+              s2[w] = xw;
+              c4 += __popcll(xw);
+              nz |= xw;
+            }
+            if (!__ballot(nz != 0)) continue;
+            Sync();
+            if (K == 5) {
+              const u64 y = LastLevel(s2, lo2, W, rows, q, lane);
+              c5 += y;
+              vk += y;
+              continue;
+            }
+            for (uint32_t c2 = lo2; c2 < W; ++c2) {
+              for (u64 m2 = First64(s2[c2]); m2; m2 &= m2 - 1) {
+                const uint32_t l = c2 * 64 + static_cast<uint32_t>(__builtin_ctzll(m2));
+                const u64* rl = rows + static_cast<uint64_t>(l) * W;
+                const uint32_t lo3 = (l + 1) >> 6;
+                u64 nz3 = 0;
+                Sync();
+                for (uint32_t w = lo3 + lane; w < W; w += kWave) {
+                  const u64 xw = s2[w] & rl[w];
+                  s3[w] = xw;
+                  c5 += __popcll(xw);
+                  nz3 |= xw;
+                }
+                if (!__ballot(nz3 != 0)) continue;
+                Sync();
+                const u64 y = LastLevel(s3, lo3, W, rows, q, lane);
+                c6 += y;
+                vk += y;
+              }
+            }
+          }
+        }
+      }
+    }
+    if (a.bk.bounds) AddBucket(a.bk, a.verts[x], vk, lane);
+  }
+  Flush(a.acc, 3, c3, lane);
+  if (K >= 4) Flush(a.acc, 4, c4, lane);
+  if (K >= 5) Flush(a.acc, 5, c5, lane);
+  if (K >= 6) Flush(a.acc, 6, c6, lane);
+}
+
+}  // namespace
+
+void LaunchCliqueSmall(const CliqueArgs& a, hipStream_t stream) {
+  if (a.v_end <= a.v_begin) return;
+  const dim3 grid(Grid(a.v_end - a.v_begin, kWaves)), block(kThreads);
+  switch (a.k) {
+    case 3: hipLaunchKernelGGL(CliqueSmallKernel<3>, grid, block, 0, stream, a); break;
+    case 4: hipLaunchKernelGGL(CliqueSmallKernel<4>, grid, block, 0, stream, a); break;
+    case 5: hipLaunchKernelGGL(CliqueSmallKernel<5>, grid, block, 0, stream, a); break;
+    case 6: hipLaunchKernelGGL(CliqueSmallKernel<6>, grid, block, 0, stream, a); break;
+  }
+}
+
+void LaunchCliqueRows(const CliqueArgs& a, hipStream_t stream) {
+  if (a.nitems == 0) return;
+  hipLaunchKernelGGL(CliqueRowsKernel, dim3(Grid(a.nitems, kWaves)), dim3(kThreads), 0, stream, a);
+}
+
+void LaunchCliqueCount(const CliqueArgs& a, hipStream_t stream) {
+  if (a.nitems == 0) return;
+  const dim3 grid(Grid(a.nitems, kWaves)), block(kThreads);
+  switch (a.k) {
+    case 3: hipLaunchKernelGGL(CliqueCountKernel<3>, grid, block, 0, stream, a); break;
+    case 4: hipLaunchKernelGGL(CliqueCountKernel<4>, grid, block, 0, stream, a); break;
+    case 5: hipLaunchKernelGGL(CliqueCountKernel<5>, grid, block, 0, stream, a); break;
+    case 6: hipLaunchKernelGGL(CliqueCountKernel<6>, grid, block, 0, stream, a); break;
+  }
+}
+
+}  // namespace sme

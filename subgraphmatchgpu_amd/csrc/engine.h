@@ -35,6 +35,9 @@ enum KernelSlot : int {
   kSlotCompact,     // candidate compaction + segment windows
   kSlotExpand,      // MATERIALIZE scatter of partial instances
   kSlotGroup,       // COMPUTE_COUNT of paths sharing a staged segment
+  kSlotCliqueSmall, // clique counting, start vertices with <= 64 upper neighbours
+  kSlotCliqueRows,  // clique counting, bit rows of larger neighbourhoods
+  kSlotCliqueCount, // clique counting over those rows
   kSlotMisc,
   kNumSlots
 };
@@ -115,6 +118,9 @@ class Context {
   //                  many values are counted together by one wave that keeps
   //                  the segment in LDS and streams each path's list
   //                  (GroupCountKernel); 0: off
+  //   clique_level_engine  0 (default): clique plans with ordering count
+  //                  each start vertex's cliques over bitmaps of its upper
+  //                  neighbourhood (clique.hip); 1: through the level engine
   struct Tuning {
     uint64_t lane_path_max = 0;
     uint32_t item_grid = 0;
@@ -123,6 +129,7 @@ class Context {
     uint32_t stream_ratio = 0;
     uint32_t group_max = 0;
     uint64_t wg_path_min = 0;
+    uint32_t clique_level_engine = 0;
   } tuning;
 
   // profiling

@@ -25,7 +25,8 @@ void Fail(Status s, const std::string& msg) { throw Error{s, msg}; }
 const char* SlotName(int slot) {
   static const char* kNames[kNumSlots] = {"prep",  "scan",   "partition", "intersect",
                                           "intersect_lane", "filter", "count", "compact",
-                                          "expand", "intersect_group", "misc"};
+                                          "expand", "intersect_group", "clique_small",
+                                          "clique_rows", "clique_count", "misc"};
   return slot >= 0 && slot < kNumSlots ? kNames[slot] : "?";
 }
 
@@ -309,6 +310,43 @@ Frame SliceFrame(const Frame& f, uint64_t l, uint64_t r) {
   return s;
 }
 
+// Timing scope around one launch.
+class Timed {
+ public:
+  Timed(Context* c, hipStream_t st, int slot, double bytes, uint64_t items)
+      : c_(c), st_(st), slot_(slot), bytes_(bytes), items_(items) {
+    if (c_->profiling) {
+      start_ = c_->TakeEvent();
+      SME_HIP(hipEventRecord(start_, st_));
+    }
+  }
+  ~Timed() {
+    if (!c_->profiling || !start_) return;
+    hipEvent_t stop = nullptr;
+    if (!c_->spare_events.empty()) {
+      stop = c_->spare_events.back();
+      c_->spare_events.pop_back();
+    } else if (hipEventCreate(&stop) != hipSuccess) {
+      c_->spare_events.push_back(start_);
+      return;
+    }
+    if (hipEventRecord(stop, st_) != hipSuccess) {
+      c_->spare_events.push_back(stop);
+      c_->spare_events.push_back(start_);
+      return;
+    }
+    c_->marks.push_back({start_, stop, slot_, bytes_, items_});
+  }
+
+ private:
+  Context* c_;
+  hipStream_t st_;
+  int slot_;
+  double bytes_;
+  uint64_t items_;
+  hipEvent_t start_ = nullptr;
+};
+
 class Runner {
  public:
   Runner(Context* ctx, const DevGraph* g, const EnginePlan* plan)
@@ -526,42 +564,6 @@ class Runner {
     return std::max<uint64_t>(per, 4096);
   }
 
-  // Timing scope around one launch.
-  class Timed {
-   public:
-    Timed(Context* c, hipStream_t st, int slot, double bytes, uint64_t items)
-        : c_(c), st_(st), slot_(slot), bytes_(bytes), items_(items) {
-      if (c_->profiling) {
-        start_ = c_->TakeEvent();
-        SME_HIP(hipEventRecord(start_, st_));
-      }
-    }
-    ~Timed() {
-      if (!c_->profiling || !start_) return;
-      hipEvent_t stop = nullptr;
-      if (!c_->spare_events.empty()) {
-        stop = c_->spare_events.back();
-        c_->spare_events.pop_back();
-      } else if (hipEventCreate(&stop) != hipSuccess) {
-        c_->spare_events.push_back(start_);
-        return;
-      }
-      if (hipEventRecord(stop, st_) != hipSuccess) {
-        c_->spare_events.push_back(stop);
-        c_->spare_events.push_back(start_);
-        return;
-      }
-      c_->marks.push_back({start_, stop, slot_, bytes_, items_});
-    }
-
-   private:
-    Context* c_;
-    hipStream_t st_;
-    int slot_;
-    double bytes_;
-    uint64_t items_;
-    hipEvent_t start_ = nullptr;
-  };
 
   void Scan(const uint64_t* in, uint64_t n, uint64_t* out) {
     Buf<uint64_t> tmp(pool_, ScanTempWords(n));
@@ -1355,6 +1357,158 @@ Status Recover(Context* ctx, Status s, const std::string& msg, std::string* err)
   return s;
 }
 
+// ---------------------------------------------------------------------------
+// Clique plans over bitmaps of upper neighbourhoods (clique.hip).  Level L of
+// a LIGHT clique plan with ordering outputs the j-cliques (j = vertices
+// matched after L: acc[nmat + 1], whatever the op) whose lowest vertex is a
+// start vertex, and its count is acc[k].
+bool LocalCliqueApplies(const Context* ctx, const EnginePlan* plan) {
+  return !ctx->tuning.clique_level_engine && plan->algo == kAlgoLight && plan->clique &&
+         plan->enable_ordering && plan->nv >= 3 && plan->nv <= kCliqueMaxK;
+}
+
+void LocalCliqueRange(Context* ctx, const DevGraph* g, const EnginePlan* plan, uint32_t vb, uint32_t ve,
+                      const BucketRef& bk, uint64_t* count, uint64_t* levels, RunStats* st) {
+  hipStream_t s = ctx->stream();
+  const int K = plan->nv;
+  const uint64_t range = ve > vb ? ve - vb : 0;
+  constexpr int kAcc = kCliqueMaxK + 1;
+  Buf<unsigned long long> acc(ctx, kAcc + 2);
+  LaunchFill64(reinterpret_cast<uint64_t*>(acc.get()), kAcc + 2, 0, s);
+  Buf<uint32_t> big(ctx, range), big_d(ctx, range), huge(ctx, range);
+  CliqueArgs a{};
+  a.rp = g->rp;
+  a.cols = g->cols;
+  a.up = g->up;
+  a.k = K;
+  a.acc = acc.get();
+  a.bk = bk;
+  a.v_begin = vb;
+  a.v_end = ve;
+  a.big = big.get();
+  a.big_d = big_d.get();
+  a.nbig = acc.get() + kAcc;
+  a.huge = huge.get();
+  a.nhuge = acc.get() + kAcc + 1;
+  {
+    Timed t(ctx, s, kSlotCliqueSmall, 20.0 * static_cast<double>(range), range);
+    LaunchCliqueSmall(a, s);
+  }
+  uint64_t* pin = ctx->pinned();
+  SME_HIP(hipMemcpyAsync(pin, acc.get() + kAcc, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  SME_HIP(hipStreamSynchronize(s));
+  st->d2h_syncs++;
+  st->batches = 1;
+  const uint64_t nbig = pin[0], nhuge = pin[1];
+  std::vector<uint32_t> hv(nbig), hd(nbig), verts(nbig), hh(nhuge);
+  std::vector<uint64_t> rf(nbig + 1, 0), wf(nbig + 1, 0);
+  if (nbig) {
+    SME_HIP(hipMemcpyAsync(hv.data(), big.get(), nbig * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    SME_HIP(hipMemcpyAsync(hd.data(), big_d.get(), nbig * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    SME_HIP(hipStreamSynchronize(s));
+    st->d2h_syncs++;
+    // largest neighbourhoods first (the append order is arbitrary)
+    std::vector<uint32_t> order(nbig);
+    for (uint64_t x = 0; x < nbig; ++x) order[x] = static_cast<uint32_t>(x);
+    std::sort(order.begin(), order.end(), [&](uint32_t p, uint32_t q) {
+      return hd[p] != hd[q] ? hd[p] > hd[q] : hv[p] < hv[q];
+    });
+    for (uint64_t x = 0; x < nbig; ++x) {
+      const uint64_t d = hd[order[x]];
+      verts[x] = hv[order[x]];
+      rf[x + 1] = rf[x] + d;
+      wf[x + 1] = wf[x] + d * ((d + 63) / 64);
+    }
+    Buf<uint32_t> dverts(ctx, nbig);
+    Buf<uint64_t> drf(ctx, nbig + 1), dwf(ctx, nbig + 1);
+    SME_HIP(hipMemcpyAsync(dverts.get(), verts.data(), nbig * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    SME_HIP(hipMemcpyAsync(drf.get(), rf.data(), (nbig + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, s));
+    SME_HIP(hipMemcpyAsync(dwf.get(), wf.data(), (nbig + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, s));
+    // batches of whole vertices whose rows fit what the run may use
+    uint64_t cap_words = static_cast<uint64_t>(0.9 * static_cast<double>(ctx->RunAvailable())) / 8;
+    if (ctx->tuning.batch_bytes) cap_words = std::min<uint64_t>(cap_words, ctx->tuning.batch_bytes / 8);
+    st->batches = 0;
+    for (uint64_t x0 = 0; x0 < nbig;) {
+      uint64_t x1 = x0 + 1;
+      while (x1 < nbig && wf[x1 + 1] - wf[x0] <= cap_words) ++x1;
+      const uint64_t words = wf[x1] - wf[x0];
+      Buf<unsigned long long> rows(ctx, words);
+      a.verts = dverts.get() + x0;
+      a.nverts = x1 - x0;
+      a.nitems = rf[x1] - rf[x0];
+      a.row_first = drf.get() + x0;
+      a.word_first = dwf.get() + x0;
+      a.rows = rows.get();
+      {
+        // per row: its vertex's list entry and split point, one key per
+        // later row, the row's words written
+        Timed t(ctx, s, kSlotCliqueRows,
+                8.0 * static_cast<double>(words) + 2.0 * static_cast<double>(a.nitems) * 20.0, a.nitems);
+        LaunchCliqueRows(a, s);
+      }
+      {
+        // every row word read at least once
+        Timed t(ctx, s, kSlotCliqueCount, 8.0 * static_cast<double>(words), a.nitems);
+        LaunchCliqueCount(a, s);
+      }
+      st->batches++;
+      x0 = x1;
+    }
+    SME_HIP(hipStreamSynchronize(s));
+  }
+  SME_HIP(hipMemcpyAsync(pin, acc.get(), kAcc * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  if (nhuge) SME_HIP(hipMemcpyAsync(hh.data(), huge.get(), nhuge * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  SME_HIP(hipStreamSynchronize(s));
+  st->d2h_syncs++;
+  uint64_t cl[kAcc];
+  std::memcpy(cl, pin, sizeof(cl));
+  uint64_t total = cl[K];
+  if (levels)
+    for (int L = 0; L < plan->nops; ++L) levels[L] = cl[std::min<int>(plan->ops[L].nmat + 1, K)];
+  // neighbourhoods beyond kLocalMax: the level engine, one run per range of
+  // consecutive such start vertices
+  std::sort(hh.begin(), hh.end());
+  for (uint64_t x = 0; x < nhuge;) {
+    uint64_t y = x + 1;
+    while (y < nhuge && hh[y] == hh[y - 1] + 1) ++y;
+    Runner r(ctx, g, plan);
+    if (bk.bounds) r.SetBuckets(bk.bounds, bk.nbuckets, bk.counts, bk.table, bk.base);
+    uint64_t c = 0, lv[kMaxOps] = {0};
+    RunStats rs;
+    r.Run(hh[x], hh[y - 1] + 1, &c, levels ? lv : nullptr, &rs);
+    total += c;
+    if (levels)
+      for (int L = 0; L < plan->nops; ++L) levels[L] += lv[L];
+    st->batches += rs.batches;
+    st->d2h_syncs += rs.d2h_syncs;
+    x = y;
+  }
+  *count = total;
+}
+
+// Start ranges whose vertex lists (12 bytes per start vertex) would take more
+// than a quarter of what the run may use are counted in consecutive parts.
+void LocalClique(Context* ctx, const DevGraph* g, const EnginePlan* plan, uint32_t vb, uint32_t ve,
+                 const BucketRef& bk, uint64_t* count, uint64_t* levels, RunStats* st) {
+  const uint64_t part = std::max<uint64_t>(1 << 12, ctx->RunAvailable() / 4 / 12);
+  *count = 0;
+  if (levels) std::fill(levels, levels + plan->nops, 0);
+  *st = RunStats{};
+  uint64_t b = vb;
+  do {
+    const uint32_t e = static_cast<uint32_t>(std::min<uint64_t>(ve, b + part));
+    uint64_t c = 0, lv[kMaxOps] = {0};
+    RunStats rs;
+    LocalCliqueRange(ctx, g, plan, static_cast<uint32_t>(b), e, bk, &c, levels ? lv : nullptr, &rs);
+    *count += c;
+    if (levels)
+      for (int L = 0; L < plan->nops; ++L) levels[L] += lv[L];
+    st->batches += rs.batches;
+    st->d2h_syncs += rs.d2h_syncs;
+    b = e;
+  } while (b < ve);
+}
+
 }  // namespace
 
 Status RunPlan(Context* ctx, const DevGraph* g, const EnginePlan* plan, uint32_t v_begin,
@@ -1385,9 +1539,13 @@ Status RunPlan(Context* ctx, const DevGraph* g, const EnginePlan* plan, uint32_t
   try {
     SME_HIP(hipSetDevice(ctx->device()));
     ctx->held_peak = ctx->held;
-    Runner r(ctx, rg, plan);
     RunStats st;
-    r.Run(v_begin, v_end, count, level_counts, &st);
+    if (LocalCliqueApplies(ctx, plan)) {
+      LocalClique(ctx, rg, plan, v_begin, v_end, BucketRef{}, count, level_counts, &st);
+    } else {
+      Runner r(ctx, rg, plan);
+      r.Run(v_begin, v_end, count, level_counts, &st);
+    }
     st.ranked = use_ranked ? 1 : 0;
     if (ctx->profiling) ctx->ResolveMarks();
     st.peak_pool_bytes = ctx->held_peak;
@@ -1471,10 +1629,20 @@ Status RunPlanBuckets(Context* ctx, const DevGraph* g, const EnginePlan* plan, c
     const bool use_table = nbuckets > 1 && 4 * range <= ctx->RunAvailable() / 4;
     Buf<uint32_t> table(ctx, use_table ? range : 0);
     if (use_table) LaunchBucketTable(dbounds.get(), nbuckets, range, table.get(), ctx->stream());
-    Runner r(ctx, g, plan);
-    r.SetBuckets(dbounds.get(), nbuckets, dcounts.get(), table.get(), bounds[0]);
     RunStats st;
-    r.Run(bounds[0], bounds[nbuckets], count, level_counts, &st);
+    if (LocalCliqueApplies(ctx, plan)) {
+      BucketRef bk{};
+      bk.bounds = dbounds.get();
+      bk.nbuckets = nbuckets;
+      bk.counts = dcounts.get();
+      bk.table = table.get();
+      bk.base = bounds[0];
+      LocalClique(ctx, g, plan, bounds[0], bounds[nbuckets], bk, count, level_counts, &st);
+    } else {
+      Runner r(ctx, g, plan);
+      r.SetBuckets(dbounds.get(), nbuckets, dcounts.get(), table.get(), bounds[0]);
+      r.Run(bounds[0], bounds[nbuckets], count, level_counts, &st);
+    }
     SME_HIP(hipMemcpyAsync(counts, dcounts.get(), nbuckets * sizeof(uint64_t), hipMemcpyDeviceToHost,
                            ctx->stream()));
     SME_HIP(hipStreamSynchronize(ctx->stream()));

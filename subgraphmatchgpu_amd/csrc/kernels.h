@@ -321,6 +321,60 @@ void LaunchSplitBatches(const uint64_t* offs, uint64_t start, uint64_t npaths,
                         uint64_t max_items, uint64_t max_paths, uint64_t* bounds,
                         uint64_t max_out, uint64_t* counters, hipStream_t stream);
 
+// ---------------------------------------------------------------------------
+// k-clique counting over bitmaps of upper neighbourhoods (clique.hip).
+//
+// With ordering, a k-clique plan counts each clique once, from its lowest
+// vertex v, and every other vertex of the clique lies in N+(v) =
+// cols[up[v], rp[v+1]) (the neighbours above v, sorted).  The count from v
+// is therefore the number of (k-1)-cliques of the graph induced on N+(v).
+// With d = |N+(v)| and N+(v) = {u_0 < ... < u_{d-1}}, that graph is held as
+// d rows of W = ceil(d/64) u64 words: bit j of row i is set when j > i and
+// u_j is a neighbour of u_i.  Cliques are then counted by AND + popcount:
+//   triangles from v   = sum_i |row_i|
+//   4-cliques from v   = sum_i sum_{j in row_i} |row_i & row_j|
+//   5-cliques from v   = sum over j in row_i, l in row_i & row_j of
+//                        |row_i & row_j & row_l|, and so on.
+// The j-cliques (j = 1..k) whose lowest vertex is counted are exactly the
+// per-level outputs of the clique plan (its level L outputs j-cliques with
+// j = matched vertices after L), so acc[j] collects them.
+constexpr uint32_t kLocalSmall = 64;    // d <= 64: one wave per start vertex, one word per row
+constexpr uint32_t kLocalMax = 16384;   // d > kLocalMax: left to the level engine
+constexpr int kCliqueMaxK = 6;
+
+struct CliqueArgs {
+  const uint64_t* rp;
+  const uint32_t* cols;
+  const uint64_t* up;
+  int k;                         // clique size, 3..kCliqueMaxK
+  unsigned long long* acc;       // acc[j] += j-cliques counted, j = 1..k (acc[0] unused)
+  // per-start-vertex buckets (bounds == nullptr: none); counts[b] += the
+  // k-cliques whose lowest vertex lies in bucket b
+  BucketRef bk;
+  // CliqueSmallKernel: start vertices [v_begin, v_end).  It counts those
+  // with d <= kLocalSmall and adds acc[1] and acc[2] (1 and d) for every
+  // vertex with d <= kLocalMax; vertices with kLocalSmall < d <= kLocalMax
+  // are appended to (big, big_d), those with d > kLocalMax to huge.
+  uint32_t v_begin, v_end;
+  uint32_t* big;
+  uint32_t* big_d;
+  unsigned long long* nbig;
+  uint32_t* huge;
+  unsigned long long* nhuge;
+  // CliqueRowsKernel / CliqueCountKernel: one batch of big vertices.
+  // Vertex t's rows are row items [row_first[t], row_first[t+1]) (one per
+  // row) and words [word_first[t], word_first[t+1]) - word_first[0] of rows.
+  const uint32_t* verts;
+  uint64_t nverts;
+  uint64_t nitems;               // row_first[nverts] - row_first[0]
+  const uint64_t* row_first;
+  const uint64_t* word_first;
+  unsigned long long* rows;
+};
+void LaunchCliqueSmall(const CliqueArgs& a, hipStream_t stream);
+void LaunchCliqueRows(const CliqueArgs& a, hipStream_t stream);
+void LaunchCliqueCount(const CliqueArgs& a, hipStream_t stream);
+
 // Exclusive scan: out[i] = sum(in[0..i)), out[n] = total.  `tmp` needs
 // ScanTempWords(n) u64 words.  out may alias in.
 size_t ScanTempWords(uint64_t n);

@@ -21,8 +21,8 @@ def emu():
     return EMU
 
 
-def run_emu(emu, path, algo, q, cl, vb, ve, budget, ordering=1, hub_min_degree=None):
-    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0")
+def run_emu(emu, path, algo, q, cl, vb, ve, budget, ordering=1, hub_min_degree=None, env=None):
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0", **(env or {}))
     cmd = [emu, path, str(algo), str(q), str(cl), str(ordering), str(vb), str(ve), str(budget)]
     if hub_min_degree is not None:
         cmd.append(str(hub_min_degree))
@@ -73,9 +73,13 @@ def test_emu_tiny_budget_invariance(emu, graphs):
         want = oracle.light_count(g.row_ptrs, g.cols, q, 3)
         seen = set()
         for budget in (40_000, 1 << 30):
-            cnt, lv, batches = run_emu(emu, p, 6, q, 3, 0, g.n, budget)
+            # batching is the level engine's (cliques too, through the
+            # level engine here)
+            cnt, lv, batches = run_emu(emu, p, 6, q, 3, 0, g.n, budget, env={"TUNE": "0,0,0,4,0,0,0,1"})
             assert (cnt, lv) == (want[0], want[1]), (q, budget)
             seen.add(batches)
+            # and the clique bitmap path under the same budget
+            assert run_emu(emu, p, 6, q, 3, 0, g.n, budget)[:2] == (want[0], want[1]), (q, budget)
         assert len(seen) > 1, "budgets did not change the batching"
 
 

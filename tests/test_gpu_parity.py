@@ -106,18 +106,23 @@ def test_cliques_closed_form(dev):
 
 
 def test_batching_invariance(fixture_graphs):
-    """A tight device budget forces many batches; counts and per-level outputs
-    must not change."""
+    """A tight device budget forces the level engine into many batches;
+    counts and per-level outputs must not change (nor with the clique
+    bitmap path under the same budget)."""
     g = fixture_graphs["rmat12"]
     big = sme.Device(0, budget_bytes=8 << 30)
     tiny = sme.Device(0, budget_bytes=3 << 20)
     gb, gt = big.upload(g), tiny.upload(g)
-    for q in (0, 2, 3, 4, 7):
-        say(f"case batching q={q}")
-        a = big.count(gb, sme.Plan(q))
-        b = tiny.count(gt, sme.Plan(q))
-        assert (a.count, a.levels) == (b.count, b.levels)
-        assert b.batches > a.batches
+    for engine in (1, 0):
+        for d in (big, tiny):
+            d.set_tuning(super_chunks=4, clique_level_engine=engine)
+        for q in (0, 2, 3, 4, 7):
+            say(f"case batching q={q} clique_level_engine={engine}")
+            a = big.count(gb, sme.Plan(q))
+            b = tiny.count(gt, sme.Plan(q))
+            assert (a.count, a.levels) == (b.count, b.levels)
+            if engine or q in (2, 4):
+                assert b.batches > a.batches
     big.close()
     tiny.close()
 
@@ -206,6 +211,8 @@ def test_self_loops_handled(dev):
                                   dict(wg_path_min=8), dict(wg_path_min=64, lane_path_max=16, group_max=8),
                                   dict(wg_path_min=2, item_grid=3)])
 def test_tunings_do_not_change_results(fixture_graphs, tune):
+    # clique plans too run through the level engine these tunables shape
+    tune = dict(tune, clique_level_engine=1)
     g = fixture_graphs["rmat10"]
     d = sme.Device(0, budget_bytes=2 << 30)
     d.set_tuning(**tune)
@@ -241,7 +248,7 @@ def test_super_chunks_with_hubs(fixture_graphs, super_chunks, stream_ratio):
     for hub in (0, 32):
         d = sme.Device(0, budget_bytes=2 << 30)
         d.set_hub_bitmaps(hub, 1 << 30)
-        d.set_tuning(super_chunks=super_chunks, stream_ratio=stream_ratio)
+        d.set_tuning(super_chunks=super_chunks, stream_ratio=stream_ratio, clique_level_engine=1)
         dg = d.upload(g)
         for q in (0, 1, 2, 3, 4, 7):
             say(f"case super_chunks={super_chunks} stream_ratio={stream_ratio} hub={hub} q={q}")
@@ -428,13 +435,14 @@ def test_config2_rmat20_full(config_counts):
     d.set_hub_bitmaps(256, 8 << 30)
     d.set_rank_order(True)
     dg = d.upload(g)
-    d.set_tuning(super_chunks=4, group_max=512)
-    for q in (sme.Q3, sme.Q2):
-        say(f"case config2 ranked q={q}")
-        r = d.count(dg, sme.Plan(q))
-        want = ent["light"][str(q)]
-        assert (r.count, r.levels) == (want["count"], want["levels"])
-        assert r.ranked == (q == sme.Q3)
+    for engine in (0, 1):
+        d.set_tuning(super_chunks=4, group_max=512, clique_level_engine=engine)
+        for q in (sme.Q3, sme.Q2):
+            say(f"case config2 ranked q={q} clique_level_engine={engine}")
+            r = d.count(dg, sme.Plan(q))
+            want = ent["light"][str(q)]
+            assert (r.count, r.levels) == (want["count"], want["levels"])
+            assert r.ranked == (q == sme.Q3)
     d.close()
 
 
@@ -453,6 +461,9 @@ def test_config3_rmat22_slices(rmat22):
         say(f"case config3 oracle q={q} range=[{a},{b})")
         r = big.count(gb, sme.Plan(q), a, b)
         assert (r.count, r.levels) == oracle.light_count(g.row_ptrs, g.cols, q, 3, v_begin=a, v_end=b)
+    # batching is the level engine's; the clique bitmap path must agree with it
+    for d in (big, small):
+        d.set_tuning(super_chunks=4, clique_level_engine=1)
     for q, (a, b) in ((sme.Q7, (3 << 20, (3 << 20) + 256)), (sme.Q4, (2500000, 2501024))):
         say(f"case config3 budgets q={q} range=[{a},{b})")
         r1 = big.count(gb, sme.Plan(q), a, b)
@@ -461,6 +472,10 @@ def test_config3_rmat22_slices(rmat22):
         assert r2.batches > r1.batches
         if q == sme.Q7:
             assert big.count(gb, sme.Plan(q, sme.GPU_GPSM_REUSE), a, b).count == r1.count
+            small.set_tuning(super_chunks=4)
+            r3 = small.count(gs, sme.Plan(q), a, b)
+            assert (r3.count, r3.levels) == (r1.count, r1.levels)
+            small.set_tuning(super_chunks=4, clique_level_engine=1)
     big.close()
     small.close()
 
@@ -488,7 +503,7 @@ def test_group_count_all_queries(ranked_dev, q):
         g, dg = graphs[name]
         want = oracle.light_count(g.row_ptrs, g.cols, q, 3)
         for gm in (8, 512):
-            d.set_tuning(super_chunks=4, group_max=gm)
+            d.set_tuning(super_chunks=4, group_max=gm, clique_level_engine=1)
             say(f"case group q={q} graph={name} group_max={gm}")
             r = d.count(dg, sme.Plan(q))
             assert (r.count, r.levels) == want, (name, gm)
@@ -509,15 +524,16 @@ def test_rank_order_counts(ranked_dev, q):
     d, graphs = ranked_dev
     g, dg = graphs["rmat12"]
     want, wl = oracle.light_count(g.row_ptrs, g.cols, q, 3)
-    d.set_tuning(super_chunks=4, group_max=512)
-    say(f"case rank q={q}")
-    r = d.count_only(dg, sme.Plan(q))
-    assert r.ranked and r.count == want
-    bounds = [0, 7, g.n // 3, g.n // 3 + 1, g.n - 50, g.n]
-    parts = [d.count_ranked(dg, sme.Plan(q), a, b) for a, b in zip(bounds[:-1], bounds[1:])]
-    assert sum(p.count for p in parts) == want
-    if q in (0, 3, 7, 11):
-        assert [sum(x) for x in zip(*[p.levels for p in parts])] == wl
+    for engine in (0, 1):
+        d.set_tuning(super_chunks=4, group_max=512, clique_level_engine=engine)
+        say(f"case rank q={q} clique_level_engine={engine}")
+        r = d.count_only(dg, sme.Plan(q))
+        assert r.ranked and r.count == want
+        bounds = [0, 7, g.n // 3, g.n // 3 + 1, g.n - 50, g.n]
+        parts = [d.count_ranked(dg, sme.Plan(q), a, b) for a, b in zip(bounds[:-1], bounds[1:])]
+        assert sum(p.count for p in parts) == want
+        if q in (0, 3, 7, 11):
+            assert [sum(x) for x in zip(*[p.levels for p in parts])] == wl
     d.set_tuning(super_chunks=4)
 
 
@@ -558,3 +574,65 @@ def test_cli_counts_and_match_list(tmp_path, fixture_graphs):
     assert int(lines[i].split(",")[0].split("=")[1]) == total
     got = [[int(x) for x in line.split()] for line in lines[i + 1:]]
     assert got == want.tolist()
+
+
+def hub_graph(n, extra_m, seed):
+    """Vertex 0 adjacent to every other vertex, plus extra_m random edges
+    among the others: vertex 0's upper neighbourhood exceeds what the clique
+    bitmap path takes (16,384), so the level engine counts it."""
+    rng = np.random.default_rng(seed)
+    adj = [set() for _ in range(n)]
+    for v in range(1, n):
+        adj[0].add(v)
+        adj[v].add(0)
+    for a, b in rng.integers(1, n, size=(extra_m, 2)):
+        if a != b:
+            adj[a].add(int(b))
+            adj[b].add(int(a))
+    return sme.from_adjacency([sorted(s) for s in adj])
+
+
+@pytest.fixture(scope="module")
+def clique_graphs(fixture_graphs):
+    # upper neighbourhoods up to 355 (rmat10, low ids are hubs), 153 (dense
+    # G(n, m)) and 16,999 (hub17k)
+    return {"rmat10": fixture_graphs["rmat10"], "er300d": sme.gen_er(300, 20000, seed=4),
+            "hub17k": hub_graph(17000, 60000, 1)}
+
+
+@pytest.mark.parametrize("q", [0, 3, 7, 11])
+def test_clique_bitmaps(clique_graphs, q):
+    """Clique plans counted over bitmaps of each start vertex's upper
+    neighbourhood (one wave per vertex up to 64 upper neighbours, bit rows in
+    device memory beyond that, the level engine past 16,384) give the
+    oracle's count and per-level outputs at every compress level, on start
+    ranges and per bucket, in several row batches under a small batch cap,
+    and the same as the level engine."""
+    d = sme.Device(0, budget_bytes=2 << 30)
+    for name, g in clique_graphs.items():
+        dg = d.upload(g)
+        d.set_tuning(super_chunks=4)
+        for cl in range(4):
+            say(f"case clique bitmaps q={q} graph={name} cl={cl}")
+            r = d.count(dg, sme.Plan(q, sme.GPU_LIGHT, cl))
+            assert (r.count, r.levels) == oracle.light_count(g.row_ptrs, g.cols, q, cl), cl
+        a, b = 1, g.n // 2
+        say(f"case clique bitmaps q={q} graph={name} range=[{a},{b})")
+        r = d.count(dg, sme.Plan(q), a, b)
+        assert (r.count, r.levels) == oracle.light_count(g.row_ptrs, g.cols, q, 3, v_begin=a, v_end=b)
+        bounds = [0, 1, 5, g.n // 3, g.n // 3 + 1, g.n - 7, g.n]
+        got, tot = d.count_buckets(dg, sme.Plan(q), bounds)
+        assert got.tolist() == [oracle.light_count(g.row_ptrs, g.cols, q, 3, v_begin=x, v_end=y)[0]
+                                for x, y in zip(bounds[:-1], bounds[1:])]
+        full = d.count(dg, sme.Plan(q))
+        assert tot.count == full.count
+        # 8 KB: less than the rows of one vertex with 355 upper neighbours
+        d.set_tuning(super_chunks=4, batch_bytes=1 << 13)
+        r = d.count(dg, sme.Plan(q))
+        assert (r.count, r.levels) == (full.count, full.levels)
+        assert r.batches > 1
+        d.set_tuning(super_chunks=4, clique_level_engine=1)
+        r = d.count(dg, sme.Plan(q))
+        assert (r.count, r.levels) == (full.count, full.levels)
+        d.set_tuning(super_chunks=4)
+    d.close()
