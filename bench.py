@@ -188,10 +188,9 @@ def main():
                     help="adjacency bitmaps for vertices of at least this degree (0: off)")
     ap.add_argument("--hub-max-gb", type=float, default=8.0)
     ap.add_argument("--workload", choices=("rmat20", "er-batch", "serve", "rmat22"), default="rmat20")
-    ap.add_argument("--r22-seconds", type=float, default=90.0,
-                    help="rmat22: enumeration seconds per pattern before the rest is projected")
-    ap.add_argument("--r22-sample-seconds", type=float, default=45.0,
-                    help="rmat22: seconds of sampled single-vertex counts for the projection")
+    ap.add_argument("--r22-check-ranges", type=int, default=16,
+                    help="rmat22: ranked ranges of 64 start vertices on which the 5-clique's level "
+                         "engine and bitmap path must agree (0: none)")
     ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-slots", type=int, default=4)
@@ -519,17 +518,16 @@ def bench_er_batch(args, sme):
 
 def bench_rmat22(args, sme):
     """Configuration 3: R-MAT scale 22, edge factor 32, 5-clique and house,
-    intermediates capped at 32 GB (--budget-gb), on the degree-ranked copy.
+    intermediates capped at 32 GB (--budget-gb), degree-ranked copy resident.
 
-    Each pattern is counted as ranked start-range tasks (sme_count_ranked)
-    in increasing rank order, with task sizes adapted to take a few seconds,
-    until --r22-seconds of enumeration.  When time runs out, the rest of the
-    ranked range is estimated from a seeded stratified sample of single
-    start vertices (--r22-sample-seconds): the projected time is the
-    measured time plus, per stratum, its size times the mean sampled time
-    per vertex.  Reported per pattern: instances counted, ranked vertices
-    covered, batches, peak pool bytes, the dominant kernel's GB/s, and the
-    projection when the count is partial."""
+    Each pattern is counted over all start vertices in one call: the
+    5-clique (with its level outputs) by bitmaps of every start vertex's
+    upper neighbourhood, the house (count only) from per-edge triangle
+    counts and a degree-ordered 4-cycle wedge walk.  Reported per pattern:
+    count, seconds, batches, peak pool bytes and every kernel family's time.
+    Beside it, the 5-clique's level engine and bitmap path count the same
+    ranked start ranges (--r22-check-ranges ranges of 64 vertices, spread
+    from the lowest to the highest degrees) and must agree."""
     import numpy as np
 
     t0 = time.perf_counter()
@@ -540,86 +538,56 @@ def bench_rmat22(args, sme):
     dev.set_rank_order(True)
     t0 = time.perf_counter()
     dg = dev.upload(g)
-    log(f"upload + ranked copy: {time.perf_counter() - t0:.1f}s")
+    upload_s = time.perf_counter() - t0
+    log(f"upload + ranked copy: {upload_s:.1f}s")
     dev.set_tuning(super_chunks=4, group_max=512)
-    rng = np.random.default_rng(SEED)
-    # work proxy of a ranked start vertex: its squared degree (ranked ids are
-    # in increasing degree order); base cuts split the proxy into 2,048 parts
-    deg = np.sort(np.diff(g.row_ptrs.astype(np.int64))).astype(np.float64)
-    cw = np.concatenate([[0.0], np.cumsum(deg * deg)])
-    base = np.unique(np.concatenate([np.searchsorted(cw, np.linspace(0, cw[-1], 2049)[1:]), [g.n]]))
-    base = base[base > 0]
     results = {}
     total_counted, total_s = 0, 0.0
     for q, name in ((7, "5-clique"), (4, "house")):
         plan = sme.Plan(q)
-        clique = q == 7
         dev.profile_reset()
         dev.profile(True)
-        count, lv, batches, peak, spent, r = 0, None, 0, 0, 0.0, 0
-        rate = None  # seconds per unit of squared degree, from the last task
-        while r < g.n and spent < args.r22_seconds:
-            # next task: the ranked vertices whose squared degrees sum to
-            # about 3 s at the last task's rate (at least one base cut)
-            if rate:
-                want = cw[r] + 3.0 / rate
-                b = int(np.searchsorted(cw, want, side="right")) - 1
-            else:
-                b = int(base[np.searchsorted(base, r, side="right")])
-            b = max(b, int(base[np.searchsorted(base, r, side="right")]))
-            b = min(g.n, max(b, r + 1))
-            res = dev.count_ranked(dg, plan, r, b, levels=clique)
-            count += res.count
-            batches += res.batches
-            peak = max(peak, res.peak_pool_bytes)
-            if clique:
-                lv = res.levels if lv is None else [x + y for x, y in zip(lv, res.levels)]
-            dt = res.elapsed_ms / 1e3
-            spent += dt
-            log(f"  {name}: ranked [{r}, {b}) {res.count} in {dt:.2f}s, {res.batches} batches "
-                f"(total {count} in {spent:.1f}s)")
-            w = cw[b] - cw[r]
-            rate = dt / w if w > 0 and dt > 0.05 else rate
-            r = b
+        res = dev.count(dg, plan) if q == 7 else dev.count_only(dg, plan)
         dev.profile(False)
-        prof = dev.profile_read()
+        prof = {k: v for k, v in dev.profile_read().items() if v["launches"]}
+        dt = res.elapsed_ms / 1e3
+        log(f"  {name}: {res.count} in {dt:.2f}s, {res.batches} batches, peak pool "
+            f"{res.peak_pool_bytes / 2**30:.2f} GiB")
         fam = max(prof, key=lambda k: prof[k]["ms"])
-        dom = prof[fam]
-        gbs = (dom["bytes"] / 1e9) / (dom["ms"] / 1e3) if dom["ms"] else 0.0
-        ent = {"query": f"Q{q} ({name})", "complete": r >= g.n, "count": count,
-               "ranked_vertices_counted": r, "seconds": spent, "batches": batches,
-               "peak_pool_gib": peak / 2**30, "rate_per_s": count / spent if spent else 0.0,
-               "dominant_kernel": {"family": fam, "ms": dom["ms"], "GB": dom["bytes"] / 1e9,
-                                   "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS}}
-        if clique:
-            ent["levels"] = lv
-        if r < g.n:
-            # stratified sample of single start vertices over [r, n)
-            strata = np.unique(np.linspace(r, g.n, 9).astype(np.int64))
-            per, est, used = [], 0.0, 0.0
-            for lo, hi in zip(strata[:-1], strata[1:]):
-                if used > args.r22_sample_seconds:
-                    break
-                vs = np.sort(rng.choice(np.arange(lo, hi), size=min(4, hi - lo), replace=False))
-                ts = []
-                for v in vs:
-                    res = dev.count_ranked(dg, plan, int(v), int(v) + 1, levels=False)
-                    ts.append(res.elapsed_ms / 1e3)
-                    used += ts[-1]
-                    log(f"  {name}: sample ranked vertex {v}: {res.count} in {ts[-1]:.2f}s")
-                per.append({"range": [int(lo), int(hi)], "sampled": len(ts), "mean_s": float(np.mean(ts))})
-                est += (hi - lo) * float(np.mean(ts))
-            sampled_all = len(per) == len(strata) - 1
-            ent["projection"] = {
-                "method": "remaining ranked range in 8 equal strata, 4 seeded random start vertices "
-                          "each counted alone; remaining time = sum of stratum size x mean time",
-                "strata": per, "complete_sample": sampled_all,
-                "remaining_seconds": est if sampled_all else None,
-                "projected_total_seconds": spent + est if sampled_all else None,
-            }
+        ent = {"query": f"Q{q} ({name})", "complete": True, "count": res.count, "seconds": dt,
+               "batches": res.batches, "peak_pool_gib": res.peak_pool_bytes / 2**30,
+               "rate_per_s": res.count / dt if dt else 0.0, "ranked_copy": bool(res.ranked),
+               "path": "clique bitmaps" if q == 7 else "edge triangle counts + 4-cycle wedges",
+               "dominant_kernel": {"family": fam, "ms": prof[fam]["ms"],
+                                   "GB": prof[fam]["bytes"] / 1e9,
+                                   "achieved_GBs": (prof[fam]["bytes"] / 1e9) / (prof[fam]["ms"] / 1e3)
+                                   if prof[fam]["ms"] else 0.0},
+               "kernel_families": {k: {"launches": v["launches"], "ms": round(v["ms"], 3),
+                                       "GB": round(v["bytes"] / 1e9, 3)} for k, v in prof.items()}}
+        ent["dominant_kernel"]["frac"] = ent["dominant_kernel"]["achieved_GBs"] / HBM_PEAK_GBS
+        if q == 7:
+            ent["levels"] = res.levels
         results[name] = ent
-        total_counted += count
-        total_s += spent
+        total_counted += res.count
+        total_s += dt
+    # the 5-clique's two engines on the same ranked start ranges
+    checks = []
+    if args.r22_check_ranges:
+        starts = np.unique(np.linspace(0, g.n - 64, args.r22_check_ranges).astype(np.int64))
+        plan = sme.Plan(7)
+        for a in starts:
+            a = int(a)
+            dev.set_tuning(super_chunks=4, group_max=512, clique_level_engine=1)
+            lvl = dev.count_ranked(dg, plan, a, a + 64)
+            dev.set_tuning(super_chunks=4, group_max=512)
+            bit = dev.count_ranked(dg, plan, a, a + 64)
+            checks.append({"ranked_range": [a, a + 64], "count": bit.count,
+                           "level_engine_ms": lvl.elapsed_ms, "bitmap_ms": bit.elapsed_ms,
+                           "agree": (lvl.count, lvl.levels) == (bit.count, bit.levels)})
+            log(f"  check [{a}, {a + 64}): {bit.count} level {lvl.elapsed_ms:.1f} ms bitmap "
+                f"{bit.elapsed_ms:.1f} ms {'agree' if checks[-1]['agree'] else 'DIFFER'}")
+        if not all(c["agree"] for c in checks):
+            raise SystemExit("5-clique: level engine and bitmap path differ")
     dev.close()
     return {
         "metric": "embeddings/sec (1 GPU, 32 GB intermediate cap)",
@@ -635,11 +603,12 @@ def bench_rmat22(args, sme):
         "dtype": "u32/u64 integer",
         "data": "synthetic (seeded R-MAT)",
         "config": {
-            "workload": f"RMAT scale 22 edge factor 32 (n={g.n}, nnz={g.nnz}); 5-clique and house, LIGHT, "
-                        "degree-ranked copy, ranked start-range tasks",
+            "workload": f"RMAT scale 22 edge factor 32 (n={g.n}, nnz={g.nnz}); 5-clique and house over all "
+                        "start vertices, degree-ranked copy",
             "budget_gb": args.budget_gb,
-            "seconds_per_pattern": args.r22_seconds,
+            "upload_s": upload_s,
             "patterns": results,
+            "five_clique_engine_checks": checks,
         },
     }
 

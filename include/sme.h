@@ -234,6 +234,14 @@ int sme_plan_vertices(const sme_plan* p);
  *                  AND + popcount over bit rows of that neighbourhood; 1:
  *                  through the level engine like every other plan.  Counts
  *                  and level outputs are identical.
+ *   house_level_engine  0 (the default): a house count (Q4, ordering on)
+ *                  over all start vertices with no level outputs asked for
+ *                  is computed from per-edge triangle counts and a
+ *                  degree-ordered 4-cycle walk: houses = sum_e t(e) p(e) -
+ *                  2 sum_e t(e)(t(e)-1), t(e) and p(e) the triangles and
+ *                  4-cycles through edge e (graphs without self-loops, every
+ *                  degree below 2^19); 1: through the level engine.  The
+ *                  count is identical.
  * Settings apply to every later call on the context; results never depend
  * on them. */
 typedef struct {
@@ -245,6 +253,7 @@ typedef struct {
   uint32_t group_max;
   uint64_t wg_path_min;
   uint32_t clique_level_engine;
+  uint32_t house_level_engine;
 } sme_tuning;
 int sme_set_tuning(sme_ctx* ctx, const sme_tuning* t);
 int sme_get_tuning(const sme_ctx* ctx, sme_tuning* t);

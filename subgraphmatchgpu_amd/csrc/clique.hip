@@ -65,8 +65,9 @@ __device__ __forceinline__ u64 First64(u64 v) {
   return (static_cast<u64>(hi) << 32) | lo;
 }
 
-// Does the sorted a[lo, hi) hold key?
-__device__ __forceinline__ bool Holds(const uint32_t* a, uint64_t lo, uint64_t hi, uint32_t key) {
+// Does the sorted a[lo, hi) hold key?  *pos: where (when it does).
+__device__ __forceinline__ bool Holds(const uint32_t* a, uint64_t lo, uint64_t hi, uint32_t key,
+                                      uint64_t* pos) {
   const uint64_t end = hi;
   while (lo < hi) {
     const uint64_t mid = (lo + hi) >> 1;
@@ -75,6 +76,7 @@ __device__ __forceinline__ bool Holds(const uint32_t* a, uint64_t lo, uint64_t h
     else
       hi = mid;
   }
+  *pos = lo;
   return lo < end && a[lo] == key;
 }
 
@@ -137,16 +139,25 @@ __global__ __launch_bounds__(kThreads) void CliqueSmallKernel(CliqueArgs a) {
     if (d < 2) continue;
     const uint32_t du = static_cast<uint32_t>(d);
     const uint32_t uj = static_cast<uint32_t>(lane) < du ? a.cols[nb + lane] : 0u;
-    u64 row = 0;  // lane i: row i
+    u64 row = 0;     // lane i: row i
+    uint32_t col = 0;  // lane j: rows i < j holding bit j
     for (uint32_t i = 0; i + 1 < du; ++i) {
       const uint32_t ui = Lane32(uj, i);
       const uint64_t lb = a.up[ui], le = a.rp[ui + 1];
+      uint64_t pos = 0;
       const bool ok = static_cast<uint32_t>(lane) > i && static_cast<uint32_t>(lane) < du &&
-                      Holds(a.cols, lb, le, uj);
+                      Holds(a.cols, lb, le, uj, &pos);
       const u64 r = __ballot(ok);
       if (static_cast<uint32_t>(lane) == i) row = r;
+      if (ok && a.tc) {
+        atomicAdd(a.tc + pos, 1u);  // edge (u_i, u_j): triangle (v, u_i, u_j)
+        ++col;
+      }
     }
     const u64 tri = __popcll(row);
+    // edge (v, u_i): the triangles (v, u_i, x) counted from v
+    if (a.tc && static_cast<uint32_t>(lane) < du && tri + col)
+      atomicAdd(a.tc + nb + lane, static_cast<uint32_t>(tri) + col);
     c3 += tri;
     u64 vk = K == 3 ? tri : 0;  // this lane's share of the k-cliques from v
     if (K >= 4) {
@@ -223,15 +234,24 @@ __global__ __launch_bounds__(kThreads) void CliqueRowsKernel(CliqueArgs a) {
     const uint32_t ui = a.cols[nb + i];
     const uint64_t lb = a.up[ui], le = a.rp[ui + 1];
     u64 keep = 0;
+    uint32_t bits = 0;
     for (uint32_t c = 0; c < W; ++c) {
       const uint32_t j = c * 64 + static_cast<uint32_t>(lane);
-      const bool ok = j > i && j < d && Holds(a.cols, lb, le, a.cols[nb + j]);
+      uint64_t pos = 0;
+      const bool ok = j > i && j < d && Holds(a.cols, lb, le, a.cols[nb + j], &pos);
       const u64 w = __ballot(ok);
       if ((c & 63u) == static_cast<uint32_t>(lane)) keep = w;
       if ((c & 63u) == 63u || c + 1 == W) {
         if (static_cast<uint32_t>(lane) <= (c & 63u)) out[(c & ~63u) + lane] = keep;
       }
+      if (ok && a.tc) {
+        atomicAdd(a.tc + pos, 1u);     // edge (u_i, u_j)
+        atomicAdd(a.tc + nb + j, 1u);  // edge (v, u_j)
+      }
+      bits += static_cast<uint32_t>(__popcll(w));
     }
+    // edge (v, u_i): one triangle per bit of row i
+    if (a.tc && lane == 0 && bits) atomicAdd(a.tc + nb + i, bits);
   }
 }
 

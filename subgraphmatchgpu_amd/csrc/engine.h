@@ -38,6 +38,8 @@ enum KernelSlot : int {
   kSlotCliqueSmall, // clique counting, start vertices with <= 64 upper neighbours
   kSlotCliqueRows,  // clique counting, bit rows of larger neighbourhoods
   kSlotCliqueCount, // clique counting over those rows
+  kSlotHouseMirror, // house: mirrored triangle counts and wedge loads
+  kSlotHouseWedge,  // house: 4-cycle wedges per apex
   kSlotMisc,
   kNumSlots
 };
@@ -121,6 +123,10 @@ class Context {
   //   clique_level_engine  0 (default): clique plans with ordering count
   //                  each start vertex's cliques over bitmaps of its upper
   //                  neighbourhood (clique.hip); 1: through the level engine
+  //   house_level_engine  0 (default): a house count over all start vertices
+  //                  without level outputs comes from per-edge triangle
+  //                  counts and a 4-cycle wedge walk (house.hip); 1: the
+  //                  level engine
   struct Tuning {
     uint64_t lane_path_max = 0;
     uint32_t item_grid = 0;
@@ -130,6 +136,7 @@ class Context {
     uint32_t group_max = 0;
     uint64_t wg_path_min = 0;
     uint32_t clique_level_engine = 0;
+    uint32_t house_level_engine = 0;
   } tuning;
 
   // profiling
@@ -173,6 +180,8 @@ class DevGraph {
   // isomorphic degree-ranked copy (RankByDegree), or nullptr
   DevGraph* ranked = nullptr;
   bool self_loops = false;
+  uint64_t max_degree = 0;     // largest |N(v)|
+  uint64_t max_up_degree = 0;  // largest |N+(v)| (neighbours above v)
   int device = -1;
   // hub adjacency bitmaps (nhub rows of hub_words u32 words)
   uint32_t* hub_id = nullptr;

@@ -26,7 +26,8 @@ const char* SlotName(int slot) {
   static const char* kNames[kNumSlots] = {"prep",  "scan",   "partition", "intersect",
                                           "intersect_lane", "filter", "count", "compact",
                                           "expand", "intersect_group", "clique_small",
-                                          "clique_rows", "clique_count", "misc"};
+                                          "clique_rows", "clique_count", "house_mirror",
+                                          "house_wedge", "misc"};
   return slot >= 0 && slot < kNumSlots ? kNames[slot] : "?";
 }
 
@@ -160,6 +161,12 @@ Status DevGraph::Upload(Context* ctx, const uint64_t* row_ptrs, const uint32_t* 
   n = nv;
   nnz = ne;
   self_loops = loops;
+  for (uint64_t v = 0; v < nv; ++v) {
+    const uint32_t* b = c + row_ptrs[v];
+    const uint32_t* e = c + row_ptrs[v + 1];
+    max_degree = std::max<uint64_t>(max_degree, e - b);
+    max_up_degree = std::max<uint64_t>(max_up_degree, e - std::upper_bound(b, e, static_cast<uint32_t>(v)));
+  }
   if (hipMalloc(reinterpret_cast<void**>(&rp), (nv + 1) * sizeof(uint64_t)) != hipSuccess ||
       hipMalloc(reinterpret_cast<void**>(&cols), (ne ? ne : 1) * sizeof(uint32_t)) != hipSuccess) {
     (void)hipGetLastError();
@@ -1368,7 +1375,8 @@ bool LocalCliqueApplies(const Context* ctx, const EnginePlan* plan) {
 }
 
 void LocalCliqueRange(Context* ctx, const DevGraph* g, const EnginePlan* plan, uint32_t vb, uint32_t ve,
-                      const BucketRef& bk, uint64_t* count, uint64_t* levels, RunStats* st) {
+                      const BucketRef& bk, uint64_t* count, uint64_t* levels, RunStats* st,
+                      uint32_t* tc = nullptr) {
   hipStream_t s = ctx->stream();
   const int K = plan->nv;
   const uint64_t range = ve > vb ? ve - vb : 0;
@@ -1390,6 +1398,7 @@ void LocalCliqueRange(Context* ctx, const DevGraph* g, const EnginePlan* plan, u
   a.nbig = acc.get() + kAcc;
   a.huge = huge.get();
   a.nhuge = acc.get() + kAcc + 1;
+  a.tc = tc;
   {
     Timed t(ctx, s, kSlotCliqueSmall, 20.0 * static_cast<double>(range), range);
     LaunchCliqueSmall(a, s);
@@ -1446,7 +1455,7 @@ void LocalCliqueRange(Context* ctx, const DevGraph* g, const EnginePlan* plan, u
                 8.0 * static_cast<double>(words) + 2.0 * static_cast<double>(a.nitems) * 20.0, a.nitems);
         LaunchCliqueRows(a, s);
       }
-      {
+      if (!tc) {
         // every row word read at least once
         Timed t(ctx, s, kSlotCliqueCount, 8.0 * static_cast<double>(words), a.nitems);
         LaunchCliqueCount(a, s);
@@ -1466,7 +1475,9 @@ void LocalCliqueRange(Context* ctx, const DevGraph* g, const EnginePlan* plan, u
   if (levels)
     for (int L = 0; L < plan->nops; ++L) levels[L] = cl[std::min<int>(plan->ops[L].nmat + 1, K)];
   // neighbourhoods beyond kLocalMax: the level engine, one run per range of
-  // consecutive such start vertices
+  // consecutive such start vertices (per-edge triangle counts have no such
+  // fallback: their callers check the graph's largest upper neighbourhood)
+  if (tc && nhuge) Fail(kErrUnsupported, "upper neighbourhood beyond the bitmap path's limit");
   std::sort(hh.begin(), hh.end());
   for (uint64_t x = 0; x < nhuge;) {
     uint64_t y = x + 1;
@@ -1489,7 +1500,8 @@ void LocalCliqueRange(Context* ctx, const DevGraph* g, const EnginePlan* plan, u
 // Start ranges whose vertex lists (12 bytes per start vertex) would take more
 // than a quarter of what the run may use are counted in consecutive parts.
 void LocalClique(Context* ctx, const DevGraph* g, const EnginePlan* plan, uint32_t vb, uint32_t ve,
-                 const BucketRef& bk, uint64_t* count, uint64_t* levels, RunStats* st) {
+                 const BucketRef& bk, uint64_t* count, uint64_t* levels, RunStats* st,
+                 uint32_t* tc = nullptr) {
   const uint64_t part = std::max<uint64_t>(1 << 12, ctx->RunAvailable() / 4 / 12);
   *count = 0;
   if (levels) std::fill(levels, levels + plan->nops, 0);
@@ -1499,7 +1511,7 @@ void LocalClique(Context* ctx, const DevGraph* g, const EnginePlan* plan, uint32
     const uint32_t e = static_cast<uint32_t>(std::min<uint64_t>(ve, b + part));
     uint64_t c = 0, lv[kMaxOps] = {0};
     RunStats rs;
-    LocalCliqueRange(ctx, g, plan, static_cast<uint32_t>(b), e, bk, &c, levels ? lv : nullptr, &rs);
+    LocalCliqueRange(ctx, g, plan, static_cast<uint32_t>(b), e, bk, &c, levels ? lv : nullptr, &rs, tc);
     *count += c;
     if (levels)
       for (int L = 0; L < plan->nops; ++L) levels[L] += lv[L];
@@ -1507,6 +1519,139 @@ void LocalClique(Context* ctx, const DevGraph* g, const EnginePlan* plan, uint32
     st->d2h_syncs += rs.d2h_syncs;
     b = e;
   } while (b < ve);
+}
+
+// ---------------------------------------------------------------------------
+// House count over all start vertices from edge statistics (house.hip).  The
+// packed accumulators hold a wedge count (at most the largest degree) above
+// bit kHouseShift and a sum of at most that many pairs of triangle counts
+// (each below the largest degree) below it, so the largest degree must stay
+// under 2^19; the per-edge triangle pass needs every upper neighbourhood
+// within the clique bitmap path's limit.
+bool LocalHouseApplies(const Context* ctx, const DevGraph* g, const EnginePlan* plan) {
+  return !ctx->tuning.house_level_engine && plan->query_type == 4 && plan->enable_ordering &&
+         !g->self_loops && g->max_degree < (1ull << 19) && g->max_up_degree <= kLocalMax;
+}
+
+void LocalHouse(Context* ctx, const DevGraph* g, uint64_t* count, RunStats* st) {
+  hipStream_t s = ctx->stream();
+  const uint64_t n = g->n, nnz = g->nnz;
+  *count = 0;
+  *st = RunStats{};
+  st->batches = 1;
+  if (n == 0 || nnz == 0) return;
+  // 1. triangles through every edge, recorded at its upper entry
+  Buf<uint32_t> t(ctx, nnz);
+  LaunchFill32(t.get(), nnz, 0, s);
+  {
+    EnginePlan tri;
+    std::string ignored;
+    if (!MakeLightEnginePlan(0, kClSpecial, true, &tri, nullptr)) Fail(kErrInvalid, "triangle plan");
+    uint64_t c = 0;
+    RunStats rs;
+    LocalClique(ctx, g, &tri, 0, static_cast<uint32_t>(n), BucketRef{}, &c, nullptr, &rs, t.get());
+    st->d2h_syncs += rs.d2h_syncs;
+  }
+  // 2. mirrored counts, wedge loads, sum of t(t-1)
+  Buf<unsigned long long> acc(ctx, 2);
+  LaunchFill64(reinterpret_cast<uint64_t*>(acc.get()), 2, 0, s);
+  Buf<unsigned long long> load(ctx, n);
+  HouseArgs h{};
+  h.rp = g->rp;
+  h.cols = g->cols;
+  h.up = g->up;
+  h.n = n;
+  h.t = t.get();
+  h.load = load.get();
+  h.acc = acc.get();
+  {
+    Timed tm(ctx, s, kSlotHouseMirror, 12.0 * static_cast<double>(nnz) + 32.0 * static_cast<double>(n), nnz);
+    LaunchHouseMirror(h, s);
+  }
+  std::vector<unsigned long long> hl(n);
+  std::vector<uint64_t> hrp(n + 1), hup(n);
+  SME_HIP(hipMemcpyAsync(hl.data(), load.get(), n * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  SME_HIP(hipMemcpyAsync(hrp.data(), g->rp, (n + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  SME_HIP(hipMemcpyAsync(hup.data(), g->up, n * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  SME_HIP(hipStreamSynchronize(s));
+  st->d2h_syncs++;
+  std::vector<uint32_t> light, heavy;
+  double wedges = 0;
+  for (uint64_t u = 0; u < n; ++u) {
+    if (!hl[u]) continue;
+    wedges += static_cast<double>(hl[u]);
+    (hl[u] <= kHouseHashMax ? light : heavy).push_back(static_cast<uint32_t>(u));
+  }
+  std::sort(heavy.begin(), heavy.end(), [&](uint32_t a, uint32_t b) {
+    return hl[a] != hl[b] ? hl[a] > hl[b] : a < b;
+  });
+  // 3a. light apexes: one wave each, LDS hash
+  if (!light.empty()) {
+    Buf<uint32_t> d(ctx, light.size());
+    SME_HIP(hipMemcpyAsync(d.get(), light.data(), light.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    h.apex = d.get();
+    h.napex = light.size();
+    double lw = 0;
+    for (uint32_t u : light) lw += static_cast<double>(hl[u]);
+    {
+      // per wedge: the entry w and its t
+      Timed tm(ctx, s, kSlotHouseWedge, 8.0 * lw, static_cast<uint64_t>(lw));
+      LaunchHouseWedgeHash(h, s);
+    }
+    SME_HIP(hipStreamSynchronize(s));
+  }
+  // 3b. heavy apexes: groups of dense per-apex arrays over w
+  if (!heavy.empty()) {
+    const uint64_t per_slot = 12 * n;  // u64 accumulator + u32 touched entry per w
+    const uint64_t fit = static_cast<uint64_t>(0.5 * static_cast<double>(ctx->RunAvailable())) / per_slot;
+    const uint64_t S = std::max<uint64_t>(1, std::min<uint64_t>({256, heavy.size(), fit}));
+    Buf<unsigned long long> slots(ctx, S * n);
+    Buf<uint32_t> touched(ctx, S * n);
+    Buf<unsigned long long> ntouched(ctx, S);
+    Buf<uint32_t> dapex(ctx, S);
+    Buf<uint64_t> dfirst(ctx, S + 1);
+    LaunchFill64(reinterpret_cast<uint64_t*>(slots.get()), S * n, 0, s);
+    LaunchFill64(reinterpret_cast<uint64_t*>(ntouched.get()), S, 0, s);
+    std::vector<uint64_t> first(S + 1);
+    for (uint64_t g0 = 0; g0 < heavy.size(); g0 += S) {
+      const uint64_t g1 = std::min<uint64_t>(heavy.size(), g0 + S);
+      const uint64_t na = g1 - g0;
+      uint64_t nfull = 0;
+      first[0] = 0;
+      double gw = 0;
+      for (uint64_t x = 0; x < na; ++x) {
+        const uint32_t u = heavy[g0 + x];
+        first[x + 1] = first[x] + (hup[u] - hrp[u]);
+        // an apex touching a large share of [0, u) is reduced by a scan
+        if (hl[u] > u / 8) nfull = x + 1;
+        gw += static_cast<double>(hl[u]);
+      }
+      SME_HIP(hipMemcpyAsync(dapex.get(), heavy.data() + g0, na * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+      SME_HIP(hipMemcpyAsync(dfirst.get(), first.data(), (na + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, s));
+      h.apex = dapex.get();
+      h.napex = na;
+      h.item_first = dfirst.get();
+      h.nitems = first[na];
+      h.nfull = nfull;
+      h.slots = slots.get();
+      h.touched = touched.get();
+      h.ntouched = ntouched.get();
+      {
+        Timed tm(ctx, s, kSlotHouseWedge, 8.0 * gw, static_cast<uint64_t>(gw));
+        LaunchHouseWedgeDense(h, s);
+        LaunchHouseWedgeReduce(h, s);
+      }
+      LaunchFill64(reinterpret_cast<uint64_t*>(ntouched.get()), na, 0, s);
+      SME_HIP(hipStreamSynchronize(s));
+      st->batches++;
+    }
+  }
+  uint64_t* pin = ctx->pinned();
+  SME_HIP(hipMemcpyAsync(pin, acc.get(), 2 * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  SME_HIP(hipStreamSynchronize(s));
+  st->d2h_syncs++;
+  (void)wedges;
+  *count = pin[0] - 2 * pin[1];
 }
 
 }  // namespace
@@ -1540,7 +1685,9 @@ Status RunPlan(Context* ctx, const DevGraph* g, const EnginePlan* plan, uint32_t
     SME_HIP(hipSetDevice(ctx->device()));
     ctx->held_peak = ctx->held;
     RunStats st;
-    if (LocalCliqueApplies(ctx, plan)) {
+    if (!ranked_ids && !level_counts && v_begin == 0 && v_end == g->n && LocalHouseApplies(ctx, rg, plan)) {
+      LocalHouse(ctx, rg, count, &st);
+    } else if (LocalCliqueApplies(ctx, plan)) {
       LocalClique(ctx, rg, plan, v_begin, v_end, BucketRef{}, count, level_counts, &st);
     } else {
       Runner r(ctx, rg, plan);

@@ -370,7 +370,56 @@ struct CliqueArgs {
   const uint64_t* row_first;
   const uint64_t* word_first;
   unsigned long long* rows;
+  // k == 3, nullable: tc[p] += the triangles through the edge stored at
+  // cols[p] (p an index of some N+(x)), recorded by CliqueSmallKernel and
+  // CliqueRowsKernel as they find each triangle (v, u_i, u_j)
+  uint32_t* tc;
 };
+// ---------------------------------------------------------------------------
+// House (5-cycle with one chord closing a triangle) counted from edge
+// statistics (house.hip).  With t(e) the triangles through edge e and p(e)
+// the 4-cycles through e, every house is a triangle on its chord e plus a
+// 4-cycle through e whose two other vertices avoid the triangle's apex:
+//   houses = sum_e t(e) p(e) - 2 sum_e t(e) (t(e) - 1).
+// sum_e t(e) p(e) is a sum over 4-cycles of the t of their four edges.  Each
+// 4-cycle has one vertex u of highest id (on a degree-ranked graph: highest
+// degree) and one vertex w opposite it; with W(u, w) the c middle vertices v
+// (v ~ u, v ~ w, v < u), the 4-cycles of (u, w) are the pairs of W and
+// together contribute (c - 1) * s, s = sum_{v in W} t(u, v) + t(v, w).  The
+// wedges u - v - w (v, w < u) are walked from each apex u and accumulated
+// per w as one packed u64 (count << kHouseShift | sum of t).
+constexpr int kHouseShift = 40;
+constexpr uint32_t kHouseHashMax = 512;  // apex loads a wave hashes in LDS
+
+struct HouseArgs {
+  const uint64_t* rp;
+  const uint32_t* cols;
+  const uint64_t* up;
+  uint64_t n;
+  uint32_t* t;                  // per CSR entry: triangles through its edge
+  unsigned long long* load;     // HouseMirrorKernel: wedges with apex u
+  unsigned long long* acc;      // acc[0] += sum (c - 1) s; acc[1] += sum_e t(e) (t(e) - 1)
+  // wedge kernels: apex vertices (HouseWedgeHashKernel: load <= kHouseHashMax)
+  const uint32_t* apex;
+  uint64_t napex;
+  // HouseWedgeDenseKernel / HouseWedgeReduceKernel: apex x accumulates in
+  // slots[x * n + w]; its items (one per lower neighbour v) are
+  // [item_first[x], item_first[x+1]).  The first nfull apexes are reduced by
+  // a scan of [0, u); the others through the list of touched w.
+  const uint64_t* item_first;
+  uint64_t nitems;
+  uint64_t nfull;
+  unsigned long long* slots;
+  uint32_t* touched;            // napex x n
+  unsigned long long* ntouched; // napex
+};
+// Lower CSR entries (v < u) take t from their mirror in N+(v); load[u] and
+// acc[1] as above.  The upper entries' t must be complete.
+void LaunchHouseMirror(const HouseArgs& a, hipStream_t stream);
+void LaunchHouseWedgeHash(const HouseArgs& a, hipStream_t stream);
+void LaunchHouseWedgeDense(const HouseArgs& a, hipStream_t stream);
+void LaunchHouseWedgeReduce(const HouseArgs& a, hipStream_t stream);
+
 void LaunchCliqueSmall(const CliqueArgs& a, hipStream_t stream);
 void LaunchCliqueRows(const CliqueArgs& a, hipStream_t stream);
 void LaunchCliqueCount(const CliqueArgs& a, hipStream_t stream);

@@ -129,8 +129,11 @@ void Launch(dim3 grid, dim3 block, const std::function<void()>& body) {
   g_block_dim = block;
   g_grid_dim = grid;
   const unsigned nwaves = (nthreads + 63) / 64;
-  for (unsigned b = 0; b < grid.x; ++b) {
-    g_block_idx = dim3(b, 0, 0);
+  const unsigned long long nblocks = 1ull * grid.x * (grid.y ? grid.y : 1) * (grid.z ? grid.z : 1);
+  for (unsigned long long b = 0; b < nblocks; ++b) {
+    // blocks in x-fastest order, as the hardware numbers them
+    g_block_idx = dim3(static_cast<unsigned>(b % grid.x), static_cast<unsigned>((b / grid.x) % (grid.y ? grid.y : 1)),
+                       static_cast<unsigned>(b / (1ull * grid.x * (grid.y ? grid.y : 1))));
     for (unsigned t = 0; t < nthreads; ++t) {
       Lane& l = g_lanes[t];
       l.tid = t;

@@ -159,6 +159,11 @@ inline unsigned int atomicCAS(unsigned int* p, unsigned int cmp, unsigned int v)
   if (o == cmp) *p = v;
   return o;
 }
+inline unsigned int atomicAdd(unsigned int* p, unsigned int v) {
+  const unsigned int o = *p;
+  *p = o + v;
+  return o;
+}
 inline unsigned long long atomicAdd(unsigned long long* p, unsigned long long v) {
   const unsigned long long o = *p;
   *p = o + v;

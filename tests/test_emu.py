@@ -34,12 +34,32 @@ def run_emu(emu, path, algo, q, cl, vb, ve, budget, ordering=1, hub_min_degree=N
     return cnt, lv, int(parts[2].split("=")[1])
 
 
+def hubs_graph(n=400, hubs=4, spokes=150, extra=1500, seed=5):
+    """A sparse G(n, m) plus a few vertices of low id joined to 150 others
+    each: upper neighbourhoods above 64 (bit rows in device memory) and
+    heavy 4-cycle apexes, with few cliques."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    adj = [set() for _ in range(n)]
+    for h in range(hubs):
+        for v in rng.choice(np.arange(hubs, n), size=spokes, replace=False):
+            adj[h].add(int(v))
+            adj[int(v)].add(h)
+    for a, b in rng.integers(0, n, size=(extra, 2)):
+        if a != b:
+            adj[a].add(int(b))
+            adj[b].add(int(a))
+    return sme.from_adjacency([sorted(x) for x in adj])
+
+
 @pytest.fixture(scope="module")
 def graphs(tmp_path_factory):
     d = tmp_path_factory.mktemp("emu")
     out = {}
     for name, g in {"rmat6": sme.gen_rmat(6, 8, seed=3), "rmat7": sme.gen_rmat(7, 6, seed=3),
-                    "er60": sme.gen_er(60, 240, seed=2), "k7": clique(7)}.items():
+                    "er60": sme.gen_er(60, 240, seed=2), "k7": clique(7),
+                    "hubs": hubs_graph()}.items():
         p = str(d / (name + ".bin"))
         sme.write_bin(p, g)
         out[name] = (g, p)
@@ -316,3 +336,33 @@ def test_pool_stream_ordered_reuse(emu):
     r = subprocess.run([os.path.join(EMU_DIR, "pool_test")], capture_output=True, text=True,
                        env=dict(os.environ, ASAN_OPTIONS="detect_leaks=0"), timeout=120)
     assert r.returncode == 0 and r.stdout.strip() == "ok", r.stdout + r.stderr[-2000:]
+
+
+@pytest.mark.parametrize("q", [0, 3, 7])
+def test_emu_clique_bitmaps(emu, graphs, q):
+    """Clique plans over bitmaps of upper neighbourhoods: counts and levels
+    against the oracle at every compress level and on a range, on a graph
+    with neighbourhoods above 64 (bit rows in device memory, several batches
+    under a small budget) and the same as the level engine."""
+    g, p = graphs["hubs"]
+    for cl in (0, 3):
+        want = oracle.light_count(g.row_ptrs, g.cols, q, cl)
+        assert run_emu(emu, p, 6, q, cl, 0, g.n, 1 << 30)[:2] == (want[0], want[1])
+    want = oracle.light_count(g.row_ptrs, g.cols, q, 3, v_begin=5, v_end=200)
+    assert run_emu(emu, p, 6, q, 3, 5, 200, 1 << 30)[:2] == (want[0], want[1])
+    cnt, lv, batches = run_emu(emu, p, 6, q, 3, 0, g.n, 1 << 30, env={"TUNE": "0,0,4096"})
+    assert batches > 1
+    assert (cnt, lv) == run_emu(emu, p, 6, q, 3, 0, g.n, 1 << 30, env={"TUNE": "0,0,0,4,0,0,0,1"})[:2]
+
+
+@pytest.mark.parametrize("name", ["rmat7", "hubs", "k7"])
+def test_emu_house_census(emu, graphs, name):
+    """The house count from per-edge triangle counts and 4-cycle wedges
+    (no level outputs, all start vertices) equals the oracle's, on the
+    original ids and the ranked copy, with several wedge groups under a
+    small budget."""
+    g, p = graphs[name]
+    want = oracle.light_count(g.row_ptrs, g.cols, 4, 3)[0]
+    for env in ({"NOLEVELS": "1"}, {"NOLEVELS": "1", "RANK": "1"}):
+        for budget in (1 << 30, 120_000):
+            assert run_emu(emu, p, 6, 4, 3, 0, g.n, budget, env=env)[0] == want, (env, budget)

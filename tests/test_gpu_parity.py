@@ -636,3 +636,48 @@ def test_clique_bitmaps(clique_graphs, q):
         assert (r.count, r.levels) == (full.count, full.levels)
         d.set_tuning(super_chunks=4)
     d.close()
+
+
+@pytest.mark.parametrize("ranked", [False, True])
+def test_house_census(fixture_graphs, ranked):
+    """A house count over all start vertices with no level outputs comes from
+    per-edge triangle counts and a degree-ordered 4-cycle wedge walk (light
+    apexes hashed in LDS, heavy ones in dense per-apex arrays, in several
+    groups under an 8 MB budget).  It equals the oracle's LIGHT count, the
+    closed form 60 C(n, 5) on K12 and the level engine's count, on the
+    original ids and on the degree-ranked copy; start ranges still go
+    through the level engine."""
+    graphs = {"rmat10": fixture_graphs["rmat10"], "rmat12": fixture_graphs["rmat12"],
+              "er300d": sme.gen_er(300, 20000, seed=4), "k12": clique(12)}
+    for budget in (2 << 30, 8 << 20):
+        d = sme.Device(0, budget_bytes=budget)
+        d.set_rank_order(ranked)
+        for name, g in graphs.items():
+            dg = d.upload(g)
+            want = 60 * 792 if name == "k12" else light_oracle(name, g, sme.Q4)[0]
+            say(f"case house census graph={name} ranked={ranked} budget={budget}")
+            r = d.count_only(dg, sme.Plan(sme.Q4))
+            assert r.count == want
+            assert r.ranked == ranked
+            d.set_tuning(super_chunks=4, house_level_engine=1)
+            assert d.count_only(dg, sme.Plan(sme.Q4)).count == want
+            d.set_tuning(super_chunks=4)
+            a, b = 3, g.n // 2
+            assert d.count_only(dg, sme.Plan(sme.Q4), a, b).count == \
+                oracle.light_count(g.row_ptrs, g.cols, sme.Q4, 3, v_begin=a, v_end=b)[0]
+            dg.release()
+        d.close()
+
+
+def test_house_census_rmat16():
+    """R-MAT scale 16, edge factor 16 (the serving graph): the house count
+    from edge statistics equals the level engine's LIGHT count."""
+    g = sme.gen_rmat(16, 16, seed=1)
+    d = sme.Device(0, budget_bytes=16 << 30)
+    d.set_rank_order(True)
+    dg = d.upload(g)
+    say("case house census rmat16")
+    r = d.count_only(dg, sme.Plan(sme.Q4))
+    d.set_tuning(super_chunks=4, house_level_engine=1)
+    assert d.count_only(dg, sme.Plan(sme.Q4)).count == r.count
+    d.close()
