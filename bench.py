@@ -144,6 +144,16 @@ def reference_cpu_config1(sme):
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "intersect_group_rmat20_traffic.json")
+KERNEL_NAMES = {
+    "intersect": "intersect (VerifyKernel)",
+    "intersect_group": "intersect_group (GroupCountKernel)",
+    "intersect_lane": "intersect_lane (PrepCountKernel)",
+    "clique_small": "clique_small (CliqueSmallKernel)",
+    "clique_rows": "clique_rows (CliqueRowsKernel)",
+    "clique_count": "clique_count (CliqueCountKernel)",
+    "house_mirror": "house_mirror (HouseMirrorKernel / EdgePairsKernel)",
+    "house_wedge": "house_wedge (HouseWedgeHashKernel / HouseWedgeDenseKernel)",
+}
 # Revision of the intersect kernel's memory behaviour; a committed traffic
 # figure is used only when it was measured on the same revision.
 KERNEL_REV = "ranked-copy-group-count-bucket-hash-tiles64"
@@ -299,9 +309,8 @@ def main():
     ms_per_step = 1000.0 * elapsed / args.steps
     value = sum(counts) * args.steps / elapsed
 
-    # the dominant intersection family: item verify kernels, group counting
-    # or lane counting, whichever took the most time
-    fam = max(("intersect", "intersect_group", "intersect_lane"), key=lambda k: prof[k]["ms"])
+    # the dominant kernel family of the step, whichever took the most time
+    fam = max((k for k in prof if k != "misc"), key=lambda k: prof[k]["ms"])
     dom = prof[fam]
 
     def family_roofline(names):
@@ -350,9 +359,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": {"intersect": "intersect (VerifyKernel)",
-                       "intersect_group": "intersect_group (GroupCountKernel)",
-                       "intersect_lane": "intersect_lane (PrepCountKernel)"}[fam],
+            "kernel": KERNEL_NAMES.get(fam, fam),
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -363,6 +370,8 @@ def main():
             "families": {
                 "intersect_all": family_roofline(("intersect", "intersect_group", "intersect_lane")),
                 "scatter": family_roofline(("expand", "compact")),
+                "clique_bitmaps": family_roofline(("clique_small", "clique_rows", "clique_count")),
+                "edge_census": family_roofline(("house_mirror", "house_wedge")),
             },
         },
     }

@@ -234,7 +234,7 @@ int sme_plan_vertices(const sme_plan* p);
  *                  AND + popcount over bit rows of that neighbourhood; 1:
  *                  through the level engine like every other plan.  Counts
  *                  and level outputs are identical.
- *   house_level_engine  0 (the default): a house count (Q4, ordering on)
+ *   census_level_engine  0 (the default): a house count (Q4, ordering on)
  *                  over all start vertices with no level outputs asked for
  *                  is computed from per-edge triangle counts and a
  *                  degree-ordered 4-cycle walk: houses = sum_e t(e) p(e) -
@@ -253,7 +253,7 @@ typedef struct {
   uint32_t group_max;
   uint64_t wg_path_min;
   uint32_t clique_level_engine;
-  uint32_t house_level_engine;
+  uint32_t census_level_engine;
 } sme_tuning;
 int sme_set_tuning(sme_ctx* ctx, const sme_tuning* t);
 int sme_get_tuning(const sme_ctx* ctx, sme_tuning* t);

@@ -58,7 +58,7 @@ class _Tuning(ctypes.Structure):
                 ("super_chunks", ctypes.c_uint32), ("batch_bytes", ctypes.c_uint64),
                 ("stream_ratio", ctypes.c_uint32), ("group_max", ctypes.c_uint32),
                 ("wg_path_min", ctypes.c_uint64), ("clique_level_engine", ctypes.c_uint32),
-                ("house_level_engine", ctypes.c_uint32)]
+                ("census_level_engine", ctypes.c_uint32)]
 
     def as_dict(self) -> dict:
         return {"lane_path_max": int(self.lane_path_max), "item_grid": int(self.item_grid),
@@ -66,7 +66,7 @@ class _Tuning(ctypes.Structure):
                 "stream_ratio": int(self.stream_ratio), "group_max": int(self.group_max),
                 "wg_path_min": int(self.wg_path_min),
                 "clique_level_engine": int(self.clique_level_engine),
-                "house_level_engine": int(self.house_level_engine)}
+                "census_level_engine": int(self.census_level_engine)}
 
 
 class _Trial(ctypes.Structure):
@@ -447,7 +447,7 @@ class Device:
 
     def set_tuning(self, lane_path_max: int = 0, item_grid: int = 0, batch_bytes: int = 0,
                    super_chunks: int = 0, stream_ratio: int = 0, group_max: int = 0,
-                   wg_path_min: int = 0, clique_level_engine: int = 0, house_level_engine: int = 0):
+                   wg_path_min: int = 0, clique_level_engine: int = 0, census_level_engine: int = 0):
         """Schedule tunables for later calls (results never depend on them):
         lane_path_max -- counting-level paths with at most this many pivot
         candidates are verified by one lane each; item_grid -- workgroup cap
@@ -464,12 +464,12 @@ class Device:
         candidates are counted by a whole workgroup each (0: off);
         clique_level_engine -- 1: clique plans run through the level engine
         instead of bitmap counting over each start vertex's upper
-        neighbourhood (0, the default); house_level_engine -- 1: a house
+        neighbourhood (0, the default); census_level_engine -- 1: a house
         count over all start vertices without level outputs runs through the
         level engine instead of per-edge triangle counts and a 4-cycle wedge
         walk (0, the default)."""
         t = _Tuning(lane_path_max, item_grid, super_chunks, batch_bytes, stream_ratio, group_max,
-                    wg_path_min, clique_level_engine, house_level_engine)
+                    wg_path_min, clique_level_engine, census_level_engine)
         _check(library().sme_set_tuning(self._h, ctypes.byref(t)), self._h)
 
     def get_tuning(self) -> dict:
@@ -569,7 +569,7 @@ class Server:
                                     t.get("super_chunks", 0), t.get("batch_bytes", 0),
                                     t.get("stream_ratio", 0), t.get("group_max", 0),
                                     t.get("wg_path_min", 0), t.get("clique_level_engine", 0),
-                                    t.get("house_level_engine", 0)))
+                                    t.get("census_level_engine", 0)))
         self._rp = np.ascontiguousarray(g.row_ptrs, dtype=np.uint64)
         self._cols = np.ascontiguousarray(g.cols, dtype=np.uint32)
         h = ctypes.c_void_p()

@@ -307,7 +307,7 @@ int sme_set_tuning(sme_ctx* ctx, const sme_tuning* t) {
   ctx->impl.tuning.group_max = t->group_max;
   ctx->impl.tuning.wg_path_min = t->wg_path_min;
   ctx->impl.tuning.clique_level_engine = t->clique_level_engine;
-  ctx->impl.tuning.house_level_engine = t->house_level_engine;
+  ctx->impl.tuning.census_level_engine = t->census_level_engine;
   return SME_OK;
 }
 
@@ -322,7 +322,7 @@ int sme_get_tuning(const sme_ctx* ctx, sme_tuning* t) {
   t->group_max = ctx->impl.tuning.group_max;
   t->wg_path_min = ctx->impl.tuning.wg_path_min;
   t->clique_level_engine = ctx->impl.tuning.clique_level_engine;
-  t->house_level_engine = ctx->impl.tuning.house_level_engine;
+  t->census_level_engine = ctx->impl.tuning.census_level_engine;
   return SME_OK;
 }
 
@@ -385,14 +385,14 @@ int sme_autotune(sme_ctx* ctx, const sme_graph* g, const sme_plan* const* plans,
     auto to_c = [](const sme::Context::Tuning& t) {
       return sme_tuning{t.lane_path_max, t.item_grid, t.super_chunks, t.batch_bytes,
                         t.stream_ratio, t.group_max, t.wg_path_min, t.clique_level_engine,
-                        t.house_level_engine};
+                        t.census_level_engine};
     };
     auto same = [](const sme::Context::Tuning& a, const sme::Context::Tuning& b) {
       return a.lane_path_max == b.lane_path_max && a.item_grid == b.item_grid &&
              a.super_chunks == b.super_chunks && a.batch_bytes == b.batch_bytes &&
              a.stream_ratio == b.stream_ratio && a.group_max == b.group_max &&
              a.wg_path_min == b.wg_path_min && a.clique_level_engine == b.clique_level_engine &&
-             a.house_level_engine == b.house_level_engine;
+             a.census_level_engine == b.census_level_engine;
     };
     const sme::Context::Tuning start = c.tuning;
     int n = 0;
@@ -488,7 +488,7 @@ int sme_server_open(int device, const uint64_t* row_ptrs, const uint32_t* cols, 
     c.tuning.group_max = cfg->tuning.group_max;
     c.tuning.wg_path_min = cfg->tuning.wg_path_min;
     c.tuning.clique_level_engine = cfg->tuning.clique_level_engine;
-    c.tuning.house_level_engine = cfg->tuning.house_level_engine;
+    c.tuning.census_level_engine = cfg->tuning.census_level_engine;
     std::string err;
     const sme::Status st = s->impl.Open(device, row_ptrs, cols, n, nnz, c, &err);
     if (st != sme::kOk) {

@@ -223,6 +223,7 @@ __global__ __launch_bounds__(kThreads) void CliqueRowsKernel(CliqueArgs a) {
   const int lane = threadIdx.x & (kWave - 1);
   const uint64_t wave = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) / kWave;
   const uint64_t nwaves = static_cast<uint64_t>(gridDim.x) * blockDim.x / kWave;
+  u64 c3 = 0;  // triangles, when no count kernel follows (per-edge counts)
   for (uint64_t t = wave; t < a.nitems; t += nwaves) {
     const uint64_t x = ItemVertex(a, t);
     const uint32_t v = a.verts[x];
@@ -252,7 +253,9 @@ __global__ __launch_bounds__(kThreads) void CliqueRowsKernel(CliqueArgs a) {
     }
     // edge (v, u_i): one triangle per bit of row i
     if (a.tc && lane == 0 && bits) atomicAdd(a.tc + nb + i, bits);
+    if (lane == 0) c3 += bits;
   }
+  if (a.tc) Flush(a.acc, 3, c3, lane);
 }
 
 // |s & row_j| over the words of row j that can be non-zero (bits above j).

@@ -123,7 +123,7 @@ class Context {
   //   clique_level_engine  0 (default): clique plans with ordering count
   //                  each start vertex's cliques over bitmaps of its upper
   //                  neighbourhood (clique.hip); 1: through the level engine
-  //   house_level_engine  0 (default): a house count over all start vertices
+  //   census_level_engine  0 (default): a house count over all start vertices
   //                  without level outputs comes from per-edge triangle
   //                  counts and a 4-cycle wedge walk (house.hip); 1: the
   //                  level engine
@@ -136,7 +136,7 @@ class Context {
     uint32_t group_max = 0;
     uint64_t wg_path_min = 0;
     uint32_t clique_level_engine = 0;
-    uint32_t house_level_engine = 0;
+    uint32_t census_level_engine = 0;
   } tuning;
 
   // profiling

@@ -1409,7 +1409,7 @@ void LocalCliqueRange(Context* ctx, const DevGraph* g, const EnginePlan* plan, u
   st->d2h_syncs++;
   st->batches = 1;
   const uint64_t nbig = pin[0], nhuge = pin[1];
-  std::vector<uint32_t> hv(nbig), hd(nbig), verts(nbig), hh(nhuge);
+  std::vector<uint32_t> hv(nbig), hd(nbig), verts(nbig), order(nbig), hh;
   std::vector<uint64_t> rf(nbig + 1, 0), wf(nbig + 1, 0);
   if (nbig) {
     SME_HIP(hipMemcpyAsync(hv.data(), big.get(), nbig * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
@@ -1417,7 +1417,6 @@ void LocalCliqueRange(Context* ctx, const DevGraph* g, const EnginePlan* plan, u
     SME_HIP(hipStreamSynchronize(s));
     st->d2h_syncs++;
     // largest neighbourhoods first (the append order is arbitrary)
-    std::vector<uint32_t> order(nbig);
     for (uint64_t x = 0; x < nbig; ++x) order[x] = static_cast<uint32_t>(x);
     std::sort(order.begin(), order.end(), [&](uint32_t p, uint32_t q) {
       return hd[p] != hd[q] ? hd[p] > hd[q] : hv[p] < hv[q];
@@ -1437,7 +1436,15 @@ void LocalCliqueRange(Context* ctx, const DevGraph* g, const EnginePlan* plan, u
     uint64_t cap_words = static_cast<uint64_t>(0.9 * static_cast<double>(ctx->RunAvailable())) / 8;
     if (ctx->tuning.batch_bytes) cap_words = std::min<uint64_t>(cap_words, ctx->tuning.batch_bytes / 8);
     st->batches = 0;
-    for (uint64_t x0 = 0; x0 < nbig;) {
+    // the largest neighbourhoods come first: any whose rows alone exceed
+    // the cap join the level engine's start vertices below
+    uint64_t first_fit = 0;
+    while (first_fit < nbig && wf[first_fit + 1] - wf[first_fit] > cap_words) ++first_fit;
+    if (first_fit) {
+      if (tc) Fail(kErrOom, "bit rows of one start vertex exceed the memory the run may use");
+      hh.insert(hh.end(), verts.begin(), verts.begin() + static_cast<int64_t>(first_fit));
+    }
+    for (uint64_t x0 = first_fit; x0 < nbig;) {
       uint64_t x1 = x0 + 1;
       while (x1 < nbig && wf[x1 + 1] - wf[x0] <= cap_words) ++x1;
       const uint64_t words = wf[x1] - wf[x0];
@@ -1449,10 +1456,16 @@ void LocalCliqueRange(Context* ctx, const DevGraph* g, const EnginePlan* plan, u
       a.word_first = dwf.get() + x0;
       a.rows = rows.get();
       {
-        // per row: its vertex's list entry and split point, one key per
-        // later row, the row's words written
+        // per row: the row vertex, its list bounds (20 bytes) and its words
+        // written; per later neighbour of the start vertex: the key and one
+        // probe of the row vertex's list (4 + 4 bytes)
+        double keys = 0;
+        for (uint64_t x = x0; x < x1; ++x) {
+          const double d = static_cast<double>(rf[x + 1] - rf[x]);
+          keys += d * (d - 1) / 2;
+        }
         Timed t(ctx, s, kSlotCliqueRows,
-                8.0 * static_cast<double>(words) + 2.0 * static_cast<double>(a.nitems) * 20.0, a.nitems);
+                8.0 * keys + 20.0 * static_cast<double>(a.nitems) + 8.0 * static_cast<double>(words), a.nitems);
         LaunchCliqueRows(a, s);
       }
       if (!tc) {
@@ -1466,11 +1479,20 @@ void LocalCliqueRange(Context* ctx, const DevGraph* g, const EnginePlan* plan, u
     SME_HIP(hipStreamSynchronize(s));
   }
   SME_HIP(hipMemcpyAsync(pin, acc.get(), kAcc * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
-  if (nhuge) SME_HIP(hipMemcpyAsync(hh.data(), huge.get(), nhuge * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  const uint64_t nfirst = hh.size();  // start vertices too large to batch
+  hh.resize(nfirst + nhuge);
+  if (nhuge)
+    SME_HIP(hipMemcpyAsync(hh.data() + nfirst, huge.get(), nhuge * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   SME_HIP(hipStreamSynchronize(s));
   st->d2h_syncs++;
   uint64_t cl[kAcc];
   std::memcpy(cl, pin, sizeof(cl));
+  // the level engine below counts those vertices whole (their 1- and
+  // 2-cliques included)
+  for (uint64_t x = 0; x < nfirst; ++x) {
+    cl[1] -= 1;
+    cl[2] -= hd[order[x]];
+  }
   uint64_t total = cl[K];
   if (levels)
     for (int L = 0; L < plan->nops; ++L) levels[L] = cl[std::min<int>(plan->ops[L].nmat + 1, K)];
@@ -1479,9 +1501,9 @@ void LocalCliqueRange(Context* ctx, const DevGraph* g, const EnginePlan* plan, u
   // fallback: their callers check the graph's largest upper neighbourhood)
   if (tc && nhuge) Fail(kErrUnsupported, "upper neighbourhood beyond the bitmap path's limit");
   std::sort(hh.begin(), hh.end());
-  for (uint64_t x = 0; x < nhuge;) {
+  for (uint64_t x = 0; x < hh.size();) {
     uint64_t y = x + 1;
-    while (y < nhuge && hh[y] == hh[y - 1] + 1) ++y;
+    while (y < hh.size() && hh[y] == hh[y - 1] + 1) ++y;
     Runner r(ctx, g, plan);
     if (bk.bounds) r.SetBuckets(bk.bounds, bk.nbuckets, bk.counts, bk.table, bk.base);
     uint64_t c = 0, lv[kMaxOps] = {0};
@@ -1502,7 +1524,7 @@ void LocalCliqueRange(Context* ctx, const DevGraph* g, const EnginePlan* plan, u
 void LocalClique(Context* ctx, const DevGraph* g, const EnginePlan* plan, uint32_t vb, uint32_t ve,
                  const BucketRef& bk, uint64_t* count, uint64_t* levels, RunStats* st,
                  uint32_t* tc = nullptr) {
-  const uint64_t part = std::max<uint64_t>(1 << 12, ctx->RunAvailable() / 4 / 12);
+  const uint64_t part = std::max<uint64_t>(64, ctx->RunAvailable() / 4 / 12);
   *count = 0;
   if (levels) std::fill(levels, levels + plan->nops, 0);
   *st = RunStats{};
@@ -1528,9 +1550,19 @@ void LocalClique(Context* ctx, const DevGraph* g, const EnginePlan* plan, uint32
 // (each below the largest degree) below it, so the largest degree must stay
 // under 2^19; the per-edge triangle pass needs every upper neighbourhood
 // within the clique bitmap path's limit.
+// Device bytes the per-edge triangle pass needs at least: the counts, the
+// start-vertex lists of one part and the bit rows of the largest upper
+// neighbourhood.
+uint64_t CensusBytes(const DevGraph* g) {
+  const uint64_t d = g->max_up_degree;
+  return 4 * g->nnz + 12 * 4096 + d * ((d + 63) / 64) * 8 + (1 << 20);
+}
+
 bool LocalHouseApplies(const Context* ctx, const DevGraph* g, const EnginePlan* plan) {
-  return !ctx->tuning.house_level_engine && plan->query_type == 4 && plan->enable_ordering &&
-         !g->self_loops && g->max_degree < (1ull << 19) && g->max_up_degree <= kLocalMax;
+  // plus the wedge loads and one dense accumulator with its touched list
+  return !ctx->tuning.census_level_engine && plan->query_type == 4 && plan->enable_ordering &&
+         !g->self_loops && g->max_degree < (1ull << 19) && g->max_up_degree <= kLocalMax &&
+         CensusBytes(g) + 20 * g->n <= ctx->RunAvailable();
 }
 
 void LocalHouse(Context* ctx, const DevGraph* g, uint64_t* count, RunStats* st) {
@@ -1654,6 +1686,69 @@ void LocalHouse(Context* ctx, const DevGraph* g, uint64_t* count, RunStats* st) 
   *count = pin[0] - 2 * pin[1];
 }
 
+
+// Diamond (Q2, LIGHT, ordering on) over all start vertices: every level
+// output is a whole-graph count -- start vertices n, edges m, triangles
+// through an edge summed over edges (3T), and the count D = sum_e
+// t(e)(t(e)-1)/2 (two triangles sharing the chord, LIGHTDirectCount.cuh:
+// 31-37) -- so it follows from the per-edge triangle counts of any
+// isomorphic copy.
+bool LocalDiamondApplies(const Context* ctx, const DevGraph* g, const EnginePlan* plan) {
+  return !ctx->tuning.census_level_engine && plan->algo == kAlgoLight && plan->query_type == 2 &&
+         plan->enable_ordering && !g->self_loops && g->max_up_degree <= kLocalMax &&
+         CensusBytes(g) <= ctx->RunAvailable();
+}
+
+void LocalDiamond(Context* ctx, const DevGraph* g, const EnginePlan* plan, uint64_t* count,
+                  uint64_t* levels, RunStats* st) {
+  hipStream_t s = ctx->stream();
+  const uint64_t n = g->n, nnz = g->nnz;
+  *st = RunStats{};
+  st->batches = 1;
+  uint64_t tri = 0, tri_levels[kMaxOps] = {0};
+  Buf<uint32_t> t(ctx, nnz);
+  Buf<unsigned long long> acc(ctx, 2);
+  LaunchFill64(reinterpret_cast<uint64_t*>(acc.get()), 2, 0, s);
+  if (nnz) {
+    LaunchFill32(t.get(), nnz, 0, s);
+    EnginePlan tp;
+    if (!MakeLightEnginePlan(0, kClSpecial, true, &tp, nullptr)) Fail(kErrInvalid, "triangle plan");
+    RunStats rs;
+    LocalClique(ctx, g, &tp, 0, static_cast<uint32_t>(n), BucketRef{}, &tri, tri_levels, &rs, t.get());
+    st->d2h_syncs += rs.d2h_syncs;
+    HouseArgs h{};
+    h.rp = g->rp;
+    h.up = g->up;
+    h.n = n;
+    h.t = t.get();
+    h.acc = acc.get();
+    Timed tm(ctx, s, kSlotHouseMirror, 4.0 * static_cast<double>(nnz) / 2 + 16.0 * static_cast<double>(n), nnz / 2);
+    LaunchEdgePairs(h, s);
+  }
+  uint64_t* pin = ctx->pinned();
+  SME_HIP(hipMemcpyAsync(pin, acc.get(), 2 * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  SME_HIP(hipStreamSynchronize(s));
+  st->d2h_syncs++;
+  const uint64_t three_t = pin[0], d = pin[1];
+  if (three_t != 3 * tri) Fail(kErrInvalid, "per-edge triangle counts disagree with the triangle count");
+  *count = d;
+  if (levels) {
+    // tri_levels of the triangle plan (M0 C1 M1 CC2): n, m, m, T
+    const uint64_t m = tri_levels[1];
+    for (int L = 0; L < plan->nops; ++L) {
+      const OpDesc& o = plan->ops[L];
+      uint64_t v = d;  // FILTER_COMPUTE, COMPUTE_COUNT, COUNT: pairs of triangles
+      if (o.kind == kMaterialize && o.u == 0)
+        v = n;
+      else if ((o.kind == kCompute || o.kind == kMaterialize) && o.u == 2)
+        v = m;
+      else if (o.kind == kCompute || o.kind == kComputePathCount || o.kind == kMaterialize)
+        v = three_t;
+      levels[L] = v;
+    }
+  }
+}
+
 }  // namespace
 
 Status RunPlan(Context* ctx, const DevGraph* g, const EnginePlan* plan, uint32_t v_begin,
@@ -1685,15 +1780,34 @@ Status RunPlan(Context* ctx, const DevGraph* g, const EnginePlan* plan, uint32_t
     SME_HIP(hipSetDevice(ctx->device()));
     ctx->held_peak = ctx->held;
     RunStats st;
-    if (!ranked_ids && !level_counts && v_begin == 0 && v_end == g->n && LocalHouseApplies(ctx, rg, plan)) {
-      LocalHouse(ctx, rg, count, &st);
+    // the diamond's level outputs are whole-graph counts too: the ranked
+    // copy serves a full-range run with them
+    const DevGraph* dg = g->ranked ? g->ranked : g;
+    const bool full = !ranked_ids && v_begin == 0 && v_end == g->n;
+    bool on_ranked = use_ranked;
+    // the edge census needs per-edge triangle counts and bit rows in memory;
+    // when the budget cannot hold them the level engine runs instead
+    auto census = [&](auto&& body) {
+      try {
+        body();
+        return true;
+      } catch (const Error& e) {
+        if (e.status != kErrOom) throw;
+        return false;
+      }
+    };
+    if (full && !level_counts && LocalHouseApplies(ctx, rg, plan) &&
+        census([&] { LocalHouse(ctx, rg, count, &st); })) {
+    } else if (full && LocalDiamondApplies(ctx, dg, plan) &&
+               census([&] { LocalDiamond(ctx, dg, plan, count, level_counts, &st); })) {
+      on_ranked = dg != g;
     } else if (LocalCliqueApplies(ctx, plan)) {
       LocalClique(ctx, rg, plan, v_begin, v_end, BucketRef{}, count, level_counts, &st);
     } else {
       Runner r(ctx, rg, plan);
       r.Run(v_begin, v_end, count, level_counts, &st);
     }
-    st.ranked = use_ranked ? 1 : 0;
+    st.ranked = on_ranked ? 1 : 0;
     if (ctx->profiling) ctx->ResolveMarks();
     st.peak_pool_bytes = ctx->held_peak;
     st.elapsed_ms =

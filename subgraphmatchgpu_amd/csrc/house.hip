@@ -102,6 +102,22 @@ __global__ __launch_bounds__(kThreads) void HouseMirrorKernel(HouseArgs a) {
   Flush(a.acc, 1, sq, lane);
 }
 
+__global__ __launch_bounds__(kThreads) void EdgePairsKernel(HouseArgs a) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t wave = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) / kWave;
+  const uint64_t nwaves = static_cast<uint64_t>(gridDim.x) * blockDim.x / kWave;
+  u64 tri = 0, pairs = 0;
+  for (uint64_t u = wave; u < a.n; u += nwaves) {
+    for (uint64_t k = a.up[u] + lane; k < a.rp[u + 1]; k += kWave) {
+      const u64 x = a.t[k];
+      tri += x;
+      pairs += x * (x ? x - 1 : 0) / 2;
+    }
+  }
+  Flush(a.acc, 0, tri, lane);
+  Flush(a.acc, 1, pairs, lane);
+}
+
 // One wave per apex u: every wedge u - v - w (v, w < u) goes into an LDS
 // hash table keyed by w (linear probing, at most half full since u's load,
 // and so its distinct w, is at most kHouseHashMax).
@@ -237,6 +253,11 @@ __global__ __launch_bounds__(kThreads) void HouseWedgeReduceKernel(HouseArgs a) 
 void LaunchHouseMirror(const HouseArgs& a, hipStream_t stream) {
   if (a.n == 0) return;
   hipLaunchKernelGGL(HouseMirrorKernel, dim3(Grid(a.n, kWaves)), dim3(kThreads), 0, stream, a);
+}
+
+void LaunchEdgePairs(const HouseArgs& a, hipStream_t stream) {
+  if (a.n == 0) return;
+  hipLaunchKernelGGL(EdgePairsKernel, dim3(Grid(a.n, kWaves)), dim3(kThreads), 0, stream, a);
 }
 
 void LaunchHouseWedgeHash(const HouseArgs& a, hipStream_t stream) {

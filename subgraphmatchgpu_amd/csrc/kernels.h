@@ -416,6 +416,9 @@ struct HouseArgs {
 // Lower CSR entries (v < u) take t from their mirror in N+(v); load[u] and
 // acc[1] as above.  The upper entries' t must be complete.
 void LaunchHouseMirror(const HouseArgs& a, hipStream_t stream);
+// Over the upper entries of every vertex: acc[0] += t, acc[1] += t(t-1)/2
+// (the diamonds on that edge: two triangles sharing it).
+void LaunchEdgePairs(const HouseArgs& a, hipStream_t stream);
 void LaunchHouseWedgeHash(const HouseArgs& a, hipStream_t stream);
 void LaunchHouseWedgeDense(const HouseArgs& a, hipStream_t stream);
 void LaunchHouseWedgeReduce(const HouseArgs& a, hipStream_t stream);

@@ -86,7 +86,7 @@ int main(int argc, char** argv) {
   // start vertices run on it; the output then says ranked=1)
   if (const char* env = std::getenv("RANK")) sme_set_rank_order(ctx, std::atoi(env));
   // TUNE="lane_path_max,item_grid,batch_bytes[,super_chunks[,stream_ratio[,group_max[,wg_path_min
-  //       [,clique_level_engine[,house_level_engine]]]]]]"
+  //       [,clique_level_engine[,census_level_engine]]]]]]"
   // sets the schedule tunables
   if (const char* env = std::getenv("TUNE")) {
     sme_tuning t{};
@@ -99,7 +99,7 @@ int main(int argc, char** argv) {
     if (*s == ',') t.group_max = static_cast<uint32_t>(std::strtoul(s + 1, &s, 10));
     if (*s == ',') t.wg_path_min = std::strtoull(s + 1, &s, 10);
     if (*s == ',') t.clique_level_engine = static_cast<uint32_t>(std::strtoul(s + 1, &s, 10));
-    if (*s == ',') t.house_level_engine = static_cast<uint32_t>(std::strtoul(s + 1, &s, 10));
+    if (*s == ',') t.census_level_engine = static_cast<uint32_t>(std::strtoul(s + 1, &s, 10));
     sme_set_tuning(ctx, &t);
   }
   sme_plan* plan = nullptr;

@@ -366,3 +366,11 @@ def test_emu_house_census(emu, graphs, name):
     for env in ({"NOLEVELS": "1"}, {"NOLEVELS": "1", "RANK": "1"}):
         for budget in (1 << 30, 120_000):
             assert run_emu(emu, p, 6, 4, 3, 0, g.n, budget, env=env)[0] == want, (env, budget)
+    # the diamond from the same per-edge triangle counts: count and every
+    # level output (whole-graph counts) at every compress level; a budget
+    # too small for the counts leaves it to the level engine
+    for cl in range(4):
+        want = oracle.light_count(g.row_ptrs, g.cols, 2, cl)
+        for env in ({}, {"RANK": "1"}):
+            for budget in (1 << 30, 30_000):
+                assert run_emu(emu, p, 6, 2, cl, 0, g.n, budget, env=env)[:2] == (want[0], want[1]), (cl, env)

@@ -115,13 +115,13 @@ def test_batching_invariance(fixture_graphs):
     gb, gt = big.upload(g), tiny.upload(g)
     for engine in (1, 0):
         for d in (big, tiny):
-            d.set_tuning(super_chunks=4, clique_level_engine=engine)
+            d.set_tuning(super_chunks=4, clique_level_engine=engine, census_level_engine=engine)
         for q in (0, 2, 3, 4, 7):
-            say(f"case batching q={q} clique_level_engine={engine}")
+            say(f"case batching q={q} level_engine={engine}")
             a = big.count(gb, sme.Plan(q))
             b = tiny.count(gt, sme.Plan(q))
             assert (a.count, a.levels) == (b.count, b.levels)
-            if engine or q in (2, 4):
+            if engine or q == 4:
                 assert b.batches > a.batches
     big.close()
     tiny.close()
@@ -212,7 +212,7 @@ def test_self_loops_handled(dev):
                                   dict(wg_path_min=2, item_grid=3)])
 def test_tunings_do_not_change_results(fixture_graphs, tune):
     # clique plans too run through the level engine these tunables shape
-    tune = dict(tune, clique_level_engine=1)
+    tune = dict(tune, clique_level_engine=1, census_level_engine=1)
     g = fixture_graphs["rmat10"]
     d = sme.Device(0, budget_bytes=2 << 30)
     d.set_tuning(**tune)
@@ -248,7 +248,8 @@ def test_super_chunks_with_hubs(fixture_graphs, super_chunks, stream_ratio):
     for hub in (0, 32):
         d = sme.Device(0, budget_bytes=2 << 30)
         d.set_hub_bitmaps(hub, 1 << 30)
-        d.set_tuning(super_chunks=super_chunks, stream_ratio=stream_ratio, clique_level_engine=1)
+        d.set_tuning(super_chunks=super_chunks, stream_ratio=stream_ratio, clique_level_engine=1,
+                     census_level_engine=1)
         dg = d.upload(g)
         for q in (0, 1, 2, 3, 4, 7):
             say(f"case super_chunks={super_chunks} stream_ratio={stream_ratio} hub={hub} q={q}")
@@ -436,13 +437,14 @@ def test_config2_rmat20_full(config_counts):
     d.set_rank_order(True)
     dg = d.upload(g)
     for engine in (0, 1):
-        d.set_tuning(super_chunks=4, group_max=512, clique_level_engine=engine)
+        d.set_tuning(super_chunks=4, group_max=512, clique_level_engine=engine, census_level_engine=engine)
         for q in (sme.Q3, sme.Q2):
             say(f"case config2 ranked q={q} clique_level_engine={engine}")
             r = d.count(dg, sme.Plan(q))
             want = ent["light"][str(q)]
             assert (r.count, r.levels) == (want["count"], want["levels"])
-            assert r.ranked == (q == sme.Q3)
+            # the diamond's census runs on the ranked copy too
+            assert r.ranked == (q == sme.Q3 or not engine)
     d.close()
 
 
@@ -503,7 +505,7 @@ def test_group_count_all_queries(ranked_dev, q):
         g, dg = graphs[name]
         want = oracle.light_count(g.row_ptrs, g.cols, q, 3)
         for gm in (8, 512):
-            d.set_tuning(super_chunks=4, group_max=gm, clique_level_engine=1)
+            d.set_tuning(super_chunks=4, group_max=gm, clique_level_engine=1, census_level_engine=1)
             say(f"case group q={q} graph={name} group_max={gm}")
             r = d.count(dg, sme.Plan(q))
             assert (r.count, r.levels) == want, (name, gm)
@@ -525,7 +527,7 @@ def test_rank_order_counts(ranked_dev, q):
     g, dg = graphs["rmat12"]
     want, wl = oracle.light_count(g.row_ptrs, g.cols, q, 3)
     for engine in (0, 1):
-        d.set_tuning(super_chunks=4, group_max=512, clique_level_engine=engine)
+        d.set_tuning(super_chunks=4, group_max=512, clique_level_engine=engine, census_level_engine=engine)
         say(f"case rank q={q} clique_level_engine={engine}")
         r = d.count_only(dg, sme.Plan(q))
         assert r.ranked and r.count == want
@@ -659,12 +661,18 @@ def test_house_census(fixture_graphs, ranked):
             r = d.count_only(dg, sme.Plan(sme.Q4))
             assert r.count == want
             assert r.ranked == ranked
-            d.set_tuning(super_chunks=4, house_level_engine=1)
+            d.set_tuning(super_chunks=4, census_level_engine=1)
             assert d.count_only(dg, sme.Plan(sme.Q4)).count == want
             d.set_tuning(super_chunks=4)
             a, b = 3, g.n // 2
             assert d.count_only(dg, sme.Plan(sme.Q4), a, b).count == \
                 oracle.light_count(g.row_ptrs, g.cols, sme.Q4, 3, v_begin=a, v_end=b)[0]
+            # the diamond from the same per-edge triangle counts, with its
+            # level outputs (whole-graph counts) at every compress level
+            for cl in range(4):
+                say(f"case diamond census graph={name} ranked={ranked} budget={budget} cl={cl}")
+                r = d.count(dg, sme.Plan(sme.Q2, sme.GPU_LIGHT, cl))
+                assert (r.count, r.levels) == light_oracle(name, g, sme.Q2, cl)
             dg.release()
         d.close()
 
@@ -678,6 +686,6 @@ def test_house_census_rmat16():
     dg = d.upload(g)
     say("case house census rmat16")
     r = d.count_only(dg, sme.Plan(sme.Q4))
-    d.set_tuning(super_chunks=4, house_level_engine=1)
+    d.set_tuning(super_chunks=4, census_level_engine=1)
     assert d.count_only(dg, sme.Plan(sme.Q4)).count == r.count
     d.close()
