@@ -26,6 +26,7 @@ constexpr int kThreads = 256;
 constexpr int kWave = 64;
 constexpr int kWaves = kThreads / kWave;
 constexpr uint32_t kWordsMax = kLocalMax / 64;  // words of the longest row
+static_assert(2 * kRowsHashSmall <= (1u << 10) && 2 * kRowsHashMax <= (1u << 13), "hash tables at most half full");
 constexpr uint64_t kMaxGrid = 16384;
 
 using u64 = unsigned long long;
@@ -230,7 +231,7 @@ __global__ __launch_bounds__(kThreads) void CliqueRowsKernel(CliqueArgs a) {
     const uint32_t d = static_cast<uint32_t>(a.row_first[x + 1] - a.row_first[x]);
     const uint32_t i = static_cast<uint32_t>(t - (a.row_first[x] - a.row_first[0]));
     const uint32_t W = (d + 63) / 64;
-    u64* out = a.rows + (a.word_first[x] - a.word_first[0]) + static_cast<uint64_t>(i) * W;
+    u64* rows = a.rows + (a.word_first[x] - a.word_first[0]);  // word w of row i at rows[w * d + i]
     const uint64_t nb = a.up[v];
     const uint32_t ui = a.cols[nb + i];
     const uint64_t lb = a.up[ui], le = a.rp[ui + 1];
@@ -243,7 +244,8 @@ __global__ __launch_bounds__(kThreads) void CliqueRowsKernel(CliqueArgs a) {
       const u64 w = __ballot(ok);
       if ((c & 63u) == static_cast<uint32_t>(lane)) keep = w;
       if ((c & 63u) == 63u || c + 1 == W) {
-        if (static_cast<uint32_t>(lane) <= (c & 63u)) out[(c & ~63u) + lane] = keep;
+        if (static_cast<uint32_t>(lane) <= (c & 63u))
+          rows[static_cast<uint64_t>((c & ~63u) + lane) * d + i] = keep;
       }
       if (ok && a.tc) {
         atomicAdd(a.tc + pos, 1u);     // edge (u_i, u_j)
@@ -258,18 +260,112 @@ __global__ __launch_bounds__(kThreads) void CliqueRowsKernel(CliqueArgs a) {
   if (a.tc) Flush(a.acc, 3, c3, lane);
 }
 
+// Chunk t of a batch: largest x with chunk_first[x] - chunk_first[0] <= t.
+__device__ __forceinline__ uint64_t ChunkVertex(const CliqueArgs& a, uint64_t t) {
+  const uint64_t base = a.chunk_first[0];
+  uint64_t lo = 0, hi = a.nverts - 1;
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi + 1) >> 1;
+    if (a.chunk_first[mid] - base <= t)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+// One workgroup per chunk of 64 rows of one vertex v.  N+(v) goes into an LDS
+// hash table (value -> local index j, at most half full); each wave then
+// builds rows of the chunk by streaming N+(u_i) up to the largest value of
+// N+(v), 64 consecutive entries per load, and setting bit j of the row (an
+// LDS OR) for every entry found.  Every entry of N+(u_i) lies above u_i, so
+// the bits set are exactly those j > i with u_j in N+(u_i).
+template <int kBits>
+__global__ __launch_bounds__(kThreads) void CliqueRowsHashKernel(CliqueArgs a) {
+  constexpr uint32_t kSlots = 1u << kBits;
+  constexpr uint32_t kEmptyKey = 0xffffffffu;
+  __shared__ uint32_t lds_key[kSlots];
+  __shared__ uint16_t lds_idx[kSlots];
+  __shared__ u64 lds_row[kWaves][kRowsHashMax / 64];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wid = threadIdx.x / kWave;
+  u64* row = lds_row[wid];
+  u64 c3 = 0;
+  for (uint64_t t = blockIdx.x; t < a.nchunks; t += gridDim.x) {
+    const uint64_t x = ChunkVertex(a, t);
+    const uint32_t v = a.verts[x];
+    const uint32_t d = static_cast<uint32_t>(a.row_first[x + 1] - a.row_first[x]);
+    const uint32_t W = (d + 63) / 64;
+    const uint32_t r0 = static_cast<uint32_t>(t - (a.chunk_first[x] - a.chunk_first[0])) * 64;
+    const uint32_t r1 = r0 + 64 < d ? r0 + 64 : d;
+    const uint64_t nb = a.up[v];
+    u64* rows = a.rows + (a.word_first[x] - a.word_first[0]);
+    __syncthreads();  // the previous chunk's lookups are done
+    for (uint32_t k = threadIdx.x; k < kSlots; k += kThreads) lds_key[k] = kEmptyKey;
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < d; j += kThreads) {
+      const uint32_t w = a.cols[nb + j];
+      uint32_t h = (w * 0x9E3779B1u) >> (32 - kBits);
+      while (atomicCAS(lds_key + h, kEmptyKey, w) != kEmptyKey) h = (h + 1) & (kSlots - 1);
+      lds_idx[h] = static_cast<uint16_t>(j);
+    }
+    __syncthreads();
+    const uint32_t vmax = a.cols[nb + d - 1];
+    for (uint32_t i = r0 + wid; i < r1; i += kWaves) {
+      const uint32_t ui = a.cols[nb + i];
+      const uint64_t lb = a.up[ui], le = a.rp[ui + 1];
+      for (uint32_t w = lane; w < W; w += kWave) row[w] = 0;
+      Sync();
+      uint32_t bits = 0;
+      for (uint64_t e0 = lb; e0 < le; e0 += kWave) {
+        // N+(u_i) is sorted: nothing from a chunk starting above the
+        // largest value of N+(v) on can be found
+        if (static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(a.cols[e0]))) > vmax) break;
+        const uint64_t e = e0 + lane;
+        if (e < le) {
+          const uint32_t w = a.cols[e];
+          uint32_t h = (w * 0x9E3779B1u) >> (32 - kBits);
+          uint32_t k;
+          while ((k = lds_key[h]) != w && k != kEmptyKey) h = (h + 1) & (kSlots - 1);
+          if (k == w) {
+            const uint32_t j = lds_idx[h];
+            atomicOr(row + (j >> 6), 1ull << (j & 63));
+            ++bits;
+            if (a.tc) {
+              atomicAdd(a.tc + e, 1u);       // edge (u_i, u_j)
+              atomicAdd(a.tc + nb + j, 1u);  // edge (v, u_j)
+            }
+          }
+        }
+      }
+      Sync();
+      for (uint32_t w = lane; w < W; w += kWave) rows[static_cast<uint64_t>(w) * d + i] = row[w];
+      const u64 tot = Sum64(bits);
+      if (lane == 0) {
+        if (a.tc && tot) atomicAdd(a.tc + nb + i, static_cast<uint32_t>(tot));  // edge (v, u_i)
+        c3 += tot;
+      }
+      Sync();
+    }
+  }
+  if (a.tc) Flush(a.acc, 3, c3, lane);
+}
+
 // |s & row_j| over the words of row j that can be non-zero (bits above j).
-__device__ __forceinline__ u64 Probe(const u64* s, uint32_t W, const u64* rows, uint32_t j) {
-  const u64* r = rows + static_cast<uint64_t>(j) * W;
+// Rows are stored word-major (word w of row j at rows[w * d + j]), so the
+// lanes of a probe round, holding nearby members j, read nearby addresses.
+__device__ __forceinline__ u64 Probe(const u64* s, uint32_t W, uint32_t d, const u64* rows, uint32_t j) {
+  const u64* r = rows + j;
   u64 x = 0;
-  for (uint32_t w = (j + 1) >> 6; w < W; ++w) x += __popcll(s[w] & r[w]);
+  for (uint32_t w = (j + 1) >> 6; w < W; ++w) x += __popcll(s[w] & r[static_cast<uint64_t>(w) * d]);
   return x;
 }
 
 // Sum over the members j of s (words [lo, W), in LDS) of |s & row_j|, as
 // per-lane partial sums.  Members are queued in LDS (q: 2 x 64 entries) so
 // every probe round has 64 of them, whatever the density of s.
-__device__ u64 LastLevel(const u64* s, uint32_t lo, uint32_t W, const u64* rows, uint32_t* q, int lane) {
+__device__ u64 LastLevel(const u64* s, uint32_t lo, uint32_t W, uint32_t d, const u64* rows, uint32_t* q,
+                         int lane) {
   const u64 below = lane ? (~0ull >> (64 - lane)) : 0ull;
   u64 sum = 0;
   uint32_t qn = 0;
@@ -280,7 +376,7 @@ __device__ u64 LastLevel(const u64* s, uint32_t lo, uint32_t W, const u64* rows,
     qn += __popcll(bits);
     if (qn >= static_cast<uint32_t>(kWave)) {
       Sync();
-      sum += Probe(s, W, rows, q[lane]);
+      sum += Probe(s, W, d, rows, q[lane]);
       const uint32_t rest = qn - kWave;
       const uint32_t carry = static_cast<uint32_t>(lane) < rest ? q[kWave + lane] : 0u;
       Sync();
@@ -291,7 +387,7 @@ __device__ u64 LastLevel(const u64* s, uint32_t lo, uint32_t W, const u64* rows,
   }
   if (qn) {
     Sync();
-    if (static_cast<uint32_t>(lane) < qn) sum += Probe(s, W, rows, q[lane]);
+    if (static_cast<uint32_t>(lane) < qn) sum += Probe(s, W, d, rows, q[lane]);
   }
   Sync();
   return sum;
@@ -321,12 +417,12 @@ __global__ __launch_bounds__(kThreads) void CliqueCountKernel(CliqueArgs a) {
     const uint32_t i = static_cast<uint32_t>(t - (a.row_first[x] - a.row_first[0]));
     const uint32_t W = (d + 63) / 64;
     const u64* rows = a.rows + (a.word_first[x] - a.word_first[0]);
-    const u64* ri = rows + static_cast<uint64_t>(i) * W;
+    const u64* ri = rows + i;
     const uint32_t lo1 = (i + 1) >> 6;
     u64 vk = 0;
     Sync();
     for (uint32_t w = lo1 + lane; w < W; w += kWave) {
-      const u64 xw = ri[w];
+      const u64 xw = ri[static_cast<uint64_t>(w) * d];
       if (K >= 4) s1[w] = xw;
       c3 += __popcll(xw);
       if (K == 3) vk += __popcll(xw);
@@ -334,19 +430,19 @@ __global__ __launch_bounds__(kThreads) void CliqueCountKernel(CliqueArgs a) {
     if (K >= 4) {
       Sync();
       if (K == 4) {
-        const u64 y = LastLevel(s1, lo1, W, rows, q, lane);
+        const u64 y = LastLevel(s1, lo1, W, d, rows, q, lane);
         c4 += y;
         vk += y;
       } else {
         for (uint32_t c = lo1; c < W; ++c) {
           for (u64 m1 = First64(s1[c]); m1; m1 &= m1 - 1) {
             const uint32_t j = c * 64 + static_cast<uint32_t>(__builtin_ctzll(m1));
-            const u64* rj = rows + static_cast<uint64_t>(j) * W;
+            const u64* rj = rows + j;
             const uint32_t lo2 = (j + 1) >> 6;
             u64 nz = 0;
             Sync();
             for (uint32_t w = lo2 + lane; w < W; w += kWave) {
-              const u64 xw = s1[w] & rj[w];
+              const u64 xw = s1[w] & rj[static_cast<uint64_t>(w) * d];
               s2[w] = xw;
               c4 += __popcll(xw);
               nz |= xw;
@@ -354,7 +450,7 @@ __global__ __launch_bounds__(kThreads) void CliqueCountKernel(CliqueArgs a) {
             if (!__ballot(nz != 0)) continue;
             Sync();
             if (K == 5) {
-              const u64 y = LastLevel(s2, lo2, W, rows, q, lane);
+              const u64 y = LastLevel(s2, lo2, W, d, rows, q, lane);
               c5 += y;
               vk += y;
               continue;
@@ -362,19 +458,19 @@ __global__ __launch_bounds__(kThreads) void CliqueCountKernel(CliqueArgs a) {
             for (uint32_t c2 = lo2; c2 < W; ++c2) {
               for (u64 m2 = First64(s2[c2]); m2; m2 &= m2 - 1) {
                 const uint32_t l = c2 * 64 + static_cast<uint32_t>(__builtin_ctzll(m2));
-                const u64* rl = rows + static_cast<uint64_t>(l) * W;
+                const u64* rl = rows + l;
                 const uint32_t lo3 = (l + 1) >> 6;
                 u64 nz3 = 0;
                 Sync();
                 for (uint32_t w = lo3 + lane; w < W; w += kWave) {
-                  const u64 xw = s2[w] & rl[w];
+                  const u64 xw = s2[w] & rl[static_cast<uint64_t>(w) * d];
                   s3[w] = xw;
                   c5 += __popcll(xw);
                   nz3 |= xw;
                 }
                 if (!__ballot(nz3 != 0)) continue;
                 Sync();
-                const u64 y = LastLevel(s3, lo3, W, rows, q, lane);
+                const u64 y = LastLevel(s3, lo3, W, d, rows, q, lane);
                 c6 += y;
                 vk += y;
               }
@@ -407,6 +503,15 @@ void LaunchCliqueSmall(const CliqueArgs& a, hipStream_t stream) {
 void LaunchCliqueRows(const CliqueArgs& a, hipStream_t stream) {
   if (a.nitems == 0) return;
   hipLaunchKernelGGL(CliqueRowsKernel, dim3(Grid(a.nitems, kWaves)), dim3(kThreads), 0, stream, a);
+}
+
+void LaunchCliqueRowsHash(const CliqueArgs& a, bool small_table, hipStream_t stream) {
+  if (a.nchunks == 0) return;
+  const dim3 grid(Grid(a.nchunks, 1)), block(kThreads);
+  if (small_table)
+    hipLaunchKernelGGL(CliqueRowsHashKernel<10>, grid, block, 0, stream, a);
+  else
+    hipLaunchKernelGGL(CliqueRowsHashKernel<13>, grid, block, 0, stream, a);
 }
 
 void LaunchCliqueCount(const CliqueArgs& a, hipStream_t stream) {

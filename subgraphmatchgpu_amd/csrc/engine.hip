@@ -1409,8 +1409,8 @@ void LocalCliqueRange(Context* ctx, const DevGraph* g, const EnginePlan* plan, u
   st->d2h_syncs++;
   st->batches = 1;
   const uint64_t nbig = pin[0], nhuge = pin[1];
-  std::vector<uint32_t> hv(nbig), hd(nbig), verts(nbig), order(nbig), hh;
-  std::vector<uint64_t> rf(nbig + 1, 0), wf(nbig + 1, 0);
+  std::vector<uint32_t> hv(nbig), hd(nbig), verts(nbig), order(nbig), hh, hd_sorted;
+  std::vector<uint64_t> rf(nbig + 1, 0), wf(nbig + 1, 0), cf(nbig + 1, 0);
   if (nbig) {
     SME_HIP(hipMemcpyAsync(hv.data(), big.get(), nbig * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     SME_HIP(hipMemcpyAsync(hd.data(), big_d.get(), nbig * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
@@ -1421,14 +1421,18 @@ void LocalCliqueRange(Context* ctx, const DevGraph* g, const EnginePlan* plan, u
     std::sort(order.begin(), order.end(), [&](uint32_t p, uint32_t q) {
       return hd[p] != hd[q] ? hd[p] > hd[q] : hv[p] < hv[q];
     });
+    hd_sorted.resize(nbig);
     for (uint64_t x = 0; x < nbig; ++x) {
       const uint64_t d = hd[order[x]];
+      hd_sorted[x] = hd[order[x]];
       verts[x] = hv[order[x]];
       rf[x + 1] = rf[x] + d;
       wf[x + 1] = wf[x] + d * ((d + 63) / 64);
+      cf[x + 1] = cf[x] + (d + 63) / 64;
     }
     Buf<uint32_t> dverts(ctx, nbig);
-    Buf<uint64_t> drf(ctx, nbig + 1), dwf(ctx, nbig + 1);
+    Buf<uint64_t> drf(ctx, nbig + 1), dwf(ctx, nbig + 1), dcf(ctx, nbig + 1);
+    SME_HIP(hipMemcpyAsync(dcf.get(), cf.data(), (nbig + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, s));
     SME_HIP(hipMemcpyAsync(dverts.get(), verts.data(), nbig * sizeof(uint32_t), hipMemcpyHostToDevice, s));
     SME_HIP(hipMemcpyAsync(drf.get(), rf.data(), (nbig + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, s));
     SME_HIP(hipMemcpyAsync(dwf.get(), wf.data(), (nbig + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, s));
@@ -1466,7 +1470,34 @@ void LocalCliqueRange(Context* ctx, const DevGraph* g, const EnginePlan* plan, u
         }
         Timed t(ctx, s, kSlotCliqueRows,
                 8.0 * keys + 20.0 * static_cast<double>(a.nitems) + 8.0 * static_cast<double>(words), a.nitems);
-        LaunchCliqueRows(a, s);
+        // the batch is in decreasing d: vertices above kRowsHashMax search,
+        // the others stream against an LDS hash (a small table up to
+        // kRowsHashSmall)
+        const uint64_t hash_from = std::partition_point(hd_sorted.begin() + x0, hd_sorted.begin() + x1,
+                                                        [](uint32_t d) { return d > kRowsHashMax; }) -
+                                   hd_sorted.begin();
+        const uint64_t small_from = std::partition_point(hd_sorted.begin() + hash_from, hd_sorted.begin() + x1,
+                                                         [](uint32_t d) { return d > kRowsHashSmall; }) -
+                                    hd_sorted.begin();
+        if (hash_from > x0) {
+          CliqueArgs b = a;
+          b.nverts = hash_from - x0;
+          b.nitems = rf[hash_from] - rf[x0];
+          LaunchCliqueRows(b, s);
+        }
+        for (int cls = 0; cls < 2; ++cls) {
+          const uint64_t xa = cls ? small_from : hash_from, xb = cls ? x1 : small_from;
+          if (xb <= xa) continue;
+          CliqueArgs b = a;
+          b.verts = dverts.get() + xa;
+          b.nverts = xb - xa;
+          b.row_first = drf.get() + xa;
+          b.word_first = dwf.get() + xa;
+          b.chunk_first = dcf.get() + xa;
+          b.nchunks = cf[xb] - cf[xa];
+          b.rows = rows.get() + (wf[xa] - wf[x0]);
+          LaunchCliqueRowsHash(b, cls == 1, s);
+        }
       }
       if (!tc) {
         // every row word read at least once

@@ -370,6 +370,11 @@ struct CliqueArgs {
   const uint64_t* row_first;
   const uint64_t* word_first;
   unsigned long long* rows;
+  // CliqueRowsKernel with a hash of N+(v) in LDS (d <= kRowsHashMax): vertex
+  // t owns chunks [chunk_first[t], chunk_first[t+1]) - chunk_first[0] of 64
+  // rows each
+  const uint64_t* chunk_first;
+  uint64_t nchunks;
   // k == 3, nullable: tc[p] += the triangles through the edge stored at
   // cols[p] (p an index of some N+(x)), recorded by CliqueSmallKernel and
   // CliqueRowsKernel as they find each triangle (v, u_i, u_j)
@@ -425,6 +430,13 @@ void LaunchHouseWedgeReduce(const HouseArgs& a, hipStream_t stream);
 
 void LaunchCliqueSmall(const CliqueArgs& a, hipStream_t stream);
 void LaunchCliqueRows(const CliqueArgs& a, hipStream_t stream);
+// Rows of vertices with d <= kRowsHashMax: a workgroup per chunk of 64 rows
+// keeps N+(v) as a hash (value -> local index) in LDS and streams each row
+// vertex's upper list, 64 consecutive entries per load; `small_table`: every
+// d <= kRowsHashSmall.
+constexpr uint32_t kRowsHashMax = 4096;
+constexpr uint32_t kRowsHashSmall = 512;
+void LaunchCliqueRowsHash(const CliqueArgs& a, bool small_table, hipStream_t stream);
 void LaunchCliqueCount(const CliqueArgs& a, hipStream_t stream);
 
 // Exclusive scan: out[i] = sum(in[0..i)), out[n] = total.  `tmp` needs

@@ -154,6 +154,11 @@ inline unsigned int atomicOr(unsigned int* p, unsigned int v) {
   *p = o | v;
   return o;
 }
+inline unsigned long long atomicOr(unsigned long long* p, unsigned long long v) {
+  const unsigned long long o = *p;
+  *p = o | v;
+  return o;
+}
 inline unsigned int atomicCAS(unsigned int* p, unsigned int cmp, unsigned int v) {
   const unsigned int o = *p;
   if (o == cmp) *p = v;
