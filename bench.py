@@ -143,27 +143,29 @@ def reference_cpu_config1(sme):
             f"triangle, CPU_WCOJ -t {threads}", "count": count, "elapsed_ms": ms, "engine_gpu_ms": gpu_ms}
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "intersect_group_rmat20_traffic.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "clique_rows_rmat20_traffic.json")
 KERNEL_NAMES = {
     "intersect": "intersect (VerifyKernel)",
     "intersect_group": "intersect_group (GroupCountKernel)",
     "intersect_lane": "intersect_lane (PrepCountKernel)",
     "clique_small": "clique_small (CliqueSmallKernel)",
-    "clique_rows": "clique_rows (CliqueRowsKernel)",
+    "clique_rows": "clique_rows (CliqueRowsHashKernel<10> + <13>; CliqueRowsKernel above 4,096)",
     "clique_count": "clique_count (CliqueCountKernel)",
     "house_mirror": "house_mirror (HouseMirrorKernel / EdgePairsKernel)",
     "house_wedge": "house_wedge (HouseWedgeHashKernel / HouseWedgeDenseKernel)",
 }
 # Revision of the intersect kernel's memory behaviour; a committed traffic
 # figure is used only when it was measured on the same revision.
-KERNEL_REV = "ranked-copy-group-count-bucket-hash-tiles64"
+KERNEL_REV = "clique-bitmaps-hash-rows-word-major"
 
 
-def summarize_pmc(fetch_csv, write_csv, calib_bytes, kernel="GroupCountKernel"):
-    """HBM bytes per dispatch of `kernel` from two rocprofv3 --pmc runs
+def summarize_pmc(fetch_csv, write_csv, calib_bytes, kernel="CliqueRowsHashKernel", per_launch=1):
+    """HBM bytes per launch of `kernel` from two rocprofv3 --pmc runs
     (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each scaled by what the
     same counter read for the calibration copy of calib_bytes (the largest
-    __amd_rocclr_copyBuffer dispatch: it reads and writes calib_bytes)."""
+    __amd_rocclr_copyBuffer dispatch: it reads and writes calib_bytes).  A
+    launch of the engine's kernel slot is `per_launch` dispatches of kernels
+    whose names contain `kernel` (the bit-row slot runs one per table size)."""
     import csv
 
     def load(path):
@@ -177,9 +179,10 @@ def summarize_pmc(fetch_csv, write_csv, calib_bytes, kernel="GroupCountKernel"):
     wc, wv = load(write_csv)
     f_factor = (calib_bytes / 1024.0) / fc
     w_factor = (calib_bytes / 1024.0) / wc
-    fetch = sum(fv) / len(fv) * 1024.0 * f_factor
-    write = sum(wv) / len(wv) * 1024.0 * w_factor
-    return {"kernel": kernel, "dispatches": len(fv), "fetch_kib_raw_per_dispatch": sum(fv) / len(fv),
+    fetch = sum(fv) / len(fv) * per_launch * 1024.0 * f_factor
+    write = sum(wv) / len(wv) * per_launch * 1024.0 * w_factor
+    return {"kernel": kernel, "dispatches": len(fv), "dispatches_per_launch": per_launch,
+            "fetch_kib_raw_per_dispatch": sum(fv) / len(fv),
             "write_kib_raw_per_dispatch": sum(wv) / len(wv), "fetch_factor": f_factor,
             "write_factor": w_factor, "fetch_bytes_per_launch": fetch, "write_bytes_per_launch": write,
             "bytes_per_launch": fetch + write}
@@ -230,8 +233,9 @@ def main():
     args = ap.parse_args()
     if args.summarize_pmc:
         f, w, mib = args.summarize_pmc[:3]
-        kernel = args.summarize_pmc[3] if len(args.summarize_pmc) > 3 else "GroupCountKernel"
-        print(json.dumps(summarize_pmc(f, w, int(mib) << 20, kernel)))
+        kernel = args.summarize_pmc[3] if len(args.summarize_pmc) > 3 else "CliqueRowsHashKernel"
+        per = int(args.summarize_pmc[4]) if len(args.summarize_pmc) > 4 else 1
+        print(json.dumps(summarize_pmc(f, w, int(mib) << 20, kernel, per)))
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU; --gpus must be 1"}))
@@ -536,7 +540,8 @@ def bench_rmat22(args, sme):
     count, seconds, batches, peak pool bytes and every kernel family's time.
     Beside it, the 5-clique's level engine and bitmap path count the same
     ranked start ranges (--r22-check-ranges ranges of 64 vertices, spread
-    from the lowest to the highest degrees) and must agree."""
+    over the 2^18 highest ranks, where nearly all 5-cliques start) and must
+    agree."""
     import numpy as np
 
     t0 = time.perf_counter()
@@ -582,7 +587,8 @@ def bench_rmat22(args, sme):
     # the 5-clique's two engines on the same ranked start ranges
     checks = []
     if args.r22_check_ranges:
-        starts = np.unique(np.linspace(0, g.n - 64, args.r22_check_ranges).astype(np.int64))
+        # the top 2^18 ranks hold nearly every 5-clique
+        starts = np.unique(np.linspace(max(0, g.n - (1 << 18)), g.n - 64, args.r22_check_ranges).astype(np.int64))
         plan = sme.Plan(7)
         for a in starts:
             a = int(a)
