@@ -120,16 +120,17 @@ __global__ __launch_bounds__(kThreads) void CliqueSmallKernel(CliqueArgs a) {
     const uint64_t nb = a.up[v];
     const uint64_t d = a.rp[v + 1] - nb;
     if (d > kLocalSmall) {
-      if (lane == 0) {
-        if (d <= kLocalMax) {
+      // (big == nullptr: the caller already holds these lists)
+      if (lane == 0 && d <= kLocalMax) {
+        if (a.big) {
           const u64 pos = atomicAdd(a.nbig, 1ull);
           a.big[pos] = v;
           a.big_d[pos] = static_cast<uint32_t>(d);
-          c1 += 1;
-          c2 += d;
-        } else {
-          a.huge[atomicAdd(a.nhuge, 1ull)] = v;
         }
+        c1 += 1;
+        c2 += d;
+      } else if (lane == 0 && a.huge) {
+        a.huge[atomicAdd(a.nhuge, 1ull)] = v;
       }
       continue;
     }

@@ -182,6 +182,18 @@ class DevGraph {
   bool self_loops = false;
   uint64_t max_degree = 0;     // largest |N(v)|
   uint64_t max_up_degree = 0;  // largest |N+(v)| (neighbours above v)
+  // Built at upload for the clique bitmap path over all start vertices:
+  // the vertices with kLocalSmall < |N+(v)| <= kLocalMax in decreasing
+  // |N+(v)| (ties by id) with prefix sums of their rows, row words and
+  // 64-row chunks (host and device copies), and those above kLocalMax.
+  struct UpperLists {
+    std::vector<uint32_t> verts, d, huge;
+    std::vector<uint64_t> rf, wf, cf;
+    uint32_t* dverts = nullptr;
+    uint64_t* drf = nullptr;
+    uint64_t* dwf = nullptr;
+    uint64_t* dcf = nullptr;
+  } upper;
   int device = -1;
   // hub adjacency bitmaps (nhub rows of hub_words u32 words)
   uint32_t* hub_id = nullptr;

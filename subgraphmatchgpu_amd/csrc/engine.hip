@@ -141,6 +141,10 @@ DevGraph::~DevGraph() {
   if (hub_id) (void)hipFree(hub_id);
   if (hub_bits) (void)hipFree(hub_bits);
   if (up) (void)hipFree(up);
+  if (upper.dverts) (void)hipFree(upper.dverts);
+  if (upper.drf) (void)hipFree(upper.drf);
+  if (upper.dwf) (void)hipFree(upper.dwf);
+  if (upper.dcf) (void)hipFree(upper.dcf);
   delete ranked;
 }
 
@@ -164,8 +168,55 @@ Status DevGraph::Upload(Context* ctx, const uint64_t* row_ptrs, const uint32_t* 
   for (uint64_t v = 0; v < nv; ++v) {
     const uint32_t* b = c + row_ptrs[v];
     const uint32_t* e = c + row_ptrs[v + 1];
+    const uint64_t dp = e - std::upper_bound(b, e, static_cast<uint32_t>(v));
     max_degree = std::max<uint64_t>(max_degree, e - b);
-    max_up_degree = std::max<uint64_t>(max_up_degree, e - std::upper_bound(b, e, static_cast<uint32_t>(v)));
+    max_up_degree = std::max<uint64_t>(max_up_degree, dp);
+    if (dp > kLocalMax) {
+      upper.huge.push_back(static_cast<uint32_t>(v));
+    } else if (dp > kLocalSmall) {
+      upper.verts.push_back(static_cast<uint32_t>(v));
+      upper.d.push_back(static_cast<uint32_t>(dp));
+    }
+  }
+  {
+    const uint64_t nb = upper.verts.size();
+    std::vector<uint32_t> order(nb);
+    for (uint64_t x = 0; x < nb; ++x) order[x] = static_cast<uint32_t>(x);
+    std::sort(order.begin(), order.end(), [&](uint32_t p, uint32_t q) {
+      return upper.d[p] != upper.d[q] ? upper.d[p] > upper.d[q] : upper.verts[p] < upper.verts[q];
+    });
+    std::vector<uint32_t> sv(nb), sd(nb);
+    upper.rf.assign(nb + 1, 0);
+    upper.wf.assign(nb + 1, 0);
+    upper.cf.assign(nb + 1, 0);
+    for (uint64_t x = 0; x < nb; ++x) {
+      sv[x] = upper.verts[order[x]];
+      sd[x] = upper.d[order[x]];
+      const uint64_t d = sd[x];
+      upper.rf[x + 1] = upper.rf[x] + d;
+      upper.wf[x + 1] = upper.wf[x] + d * ((d + 63) / 64);
+      upper.cf[x + 1] = upper.cf[x] + (d + 63) / 64;
+    }
+    upper.verts.swap(sv);
+    upper.d.swap(sd);
+    if (nb) {
+      if (hipMalloc(reinterpret_cast<void**>(&upper.dverts), nb * sizeof(uint32_t)) != hipSuccess ||
+          hipMalloc(reinterpret_cast<void**>(&upper.drf), (nb + 1) * sizeof(uint64_t)) != hipSuccess ||
+          hipMalloc(reinterpret_cast<void**>(&upper.dwf), (nb + 1) * sizeof(uint64_t)) != hipSuccess ||
+          hipMalloc(reinterpret_cast<void**>(&upper.dcf), (nb + 1) * sizeof(uint64_t)) != hipSuccess) {
+        (void)hipGetLastError();
+        *err = "hipMalloc of the upper-neighbourhood lists failed";
+        return kErrOom;
+      }
+      SME_HIP(hipMemcpyAsync(upper.dverts, upper.verts.data(), nb * sizeof(uint32_t), hipMemcpyHostToDevice,
+                             ctx->stream()));
+      SME_HIP(hipMemcpyAsync(upper.drf, upper.rf.data(), (nb + 1) * sizeof(uint64_t), hipMemcpyHostToDevice,
+                             ctx->stream()));
+      SME_HIP(hipMemcpyAsync(upper.dwf, upper.wf.data(), (nb + 1) * sizeof(uint64_t), hipMemcpyHostToDevice,
+                             ctx->stream()));
+      SME_HIP(hipMemcpyAsync(upper.dcf, upper.cf.data(), (nb + 1) * sizeof(uint64_t), hipMemcpyHostToDevice,
+                             ctx->stream()));
+    }
   }
   if (hipMalloc(reinterpret_cast<void**>(&rp), (nv + 1) * sizeof(uint64_t)) != hipSuccess ||
       hipMalloc(reinterpret_cast<void**>(&cols), (ne ? ne : 1) * sizeof(uint32_t)) != hipSuccess) {
@@ -1383,7 +1434,10 @@ void LocalCliqueRange(Context* ctx, const DevGraph* g, const EnginePlan* plan, u
   constexpr int kAcc = kCliqueMaxK + 1;
   Buf<unsigned long long> acc(ctx, kAcc + 2);
   LaunchFill64(reinterpret_cast<uint64_t*>(acc.get()), kAcc + 2, 0, s);
-  Buf<uint32_t> big(ctx, range), big_d(ctx, range), huge(ctx, range);
+  // over all start vertices the upload-time lists serve; otherwise the small
+  // kernel appends the range's larger vertices and they are sorted here
+  const bool whole = vb == 0 && ve == g->n;
+  Buf<uint32_t> big(ctx, whole ? 0 : range), big_d(ctx, whole ? 0 : range), huge(ctx, whole ? 0 : range);
   CliqueArgs a{};
   a.rp = g->rp;
   a.cols = g->cols;
@@ -1404,38 +1458,64 @@ void LocalCliqueRange(Context* ctx, const DevGraph* g, const EnginePlan* plan, u
     LaunchCliqueSmall(a, s);
   }
   uint64_t* pin = ctx->pinned();
-  SME_HIP(hipMemcpyAsync(pin, acc.get() + kAcc, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
-  SME_HIP(hipStreamSynchronize(s));
-  st->d2h_syncs++;
   st->batches = 1;
-  const uint64_t nbig = pin[0], nhuge = pin[1];
-  std::vector<uint32_t> hv(nbig), hd(nbig), verts(nbig), order(nbig), hh, hd_sorted;
-  std::vector<uint64_t> rf(nbig + 1, 0), wf(nbig + 1, 0), cf(nbig + 1, 0);
-  if (nbig) {
+  uint64_t nbig = 0, nhuge = 0;
+  if (whole) {
+    nbig = g->upper.verts.size();
+    nhuge = g->upper.huge.size();
+  } else {
+    SME_HIP(hipMemcpyAsync(pin, acc.get() + kAcc, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    SME_HIP(hipStreamSynchronize(s));
+    st->d2h_syncs++;
+    nbig = pin[0];
+    nhuge = pin[1];
+  }
+  std::vector<uint32_t> hv, hd, verts_local, hh, hd_local;
+  std::vector<uint64_t> rf_local, wf_local, cf_local;
+  const std::vector<uint32_t>& verts = whole ? g->upper.verts : verts_local;
+  const std::vector<uint32_t>& hd_sorted = whole ? g->upper.d : hd_local;
+  const std::vector<uint64_t>& rf = whole ? g->upper.rf : rf_local;
+  const std::vector<uint64_t>& wf = whole ? g->upper.wf : wf_local;
+  const std::vector<uint64_t>& cf = whole ? g->upper.cf : cf_local;
+  Buf<uint32_t> dverts_local(ctx, whole ? 0 : nbig);
+  Buf<uint64_t> drf_local(ctx, whole || !nbig ? 0 : nbig + 1), dwf_local(ctx, whole || !nbig ? 0 : nbig + 1),
+      dcf_local(ctx, whole || !nbig ? 0 : nbig + 1);
+  const uint32_t* dverts = whole ? g->upper.dverts : dverts_local.get();
+  const uint64_t* drf = whole ? g->upper.drf : drf_local.get();
+  const uint64_t* dwf = whole ? g->upper.dwf : dwf_local.get();
+  const uint64_t* dcf = whole ? g->upper.dcf : dcf_local.get();
+  if (nbig && !whole) {
+    hv.resize(nbig);
+    hd.resize(nbig);
     SME_HIP(hipMemcpyAsync(hv.data(), big.get(), nbig * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     SME_HIP(hipMemcpyAsync(hd.data(), big_d.get(), nbig * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     SME_HIP(hipStreamSynchronize(s));
     st->d2h_syncs++;
     // largest neighbourhoods first (the append order is arbitrary)
+    std::vector<uint32_t> order(nbig);
     for (uint64_t x = 0; x < nbig; ++x) order[x] = static_cast<uint32_t>(x);
     std::sort(order.begin(), order.end(), [&](uint32_t p, uint32_t q) {
       return hd[p] != hd[q] ? hd[p] > hd[q] : hv[p] < hv[q];
     });
-    hd_sorted.resize(nbig);
+    hd_local.resize(nbig);
+    verts_local.resize(nbig);
+    rf_local.assign(nbig + 1, 0);
+    wf_local.assign(nbig + 1, 0);
+    cf_local.assign(nbig + 1, 0);
     for (uint64_t x = 0; x < nbig; ++x) {
       const uint64_t d = hd[order[x]];
-      hd_sorted[x] = hd[order[x]];
-      verts[x] = hv[order[x]];
-      rf[x + 1] = rf[x] + d;
-      wf[x + 1] = wf[x] + d * ((d + 63) / 64);
-      cf[x + 1] = cf[x] + (d + 63) / 64;
+      hd_local[x] = hd[order[x]];
+      verts_local[x] = hv[order[x]];
+      rf_local[x + 1] = rf_local[x] + d;
+      wf_local[x + 1] = wf_local[x] + d * ((d + 63) / 64);
+      cf_local[x + 1] = cf_local[x] + (d + 63) / 64;
     }
-    Buf<uint32_t> dverts(ctx, nbig);
-    Buf<uint64_t> drf(ctx, nbig + 1), dwf(ctx, nbig + 1), dcf(ctx, nbig + 1);
-    SME_HIP(hipMemcpyAsync(dcf.get(), cf.data(), (nbig + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, s));
-    SME_HIP(hipMemcpyAsync(dverts.get(), verts.data(), nbig * sizeof(uint32_t), hipMemcpyHostToDevice, s));
-    SME_HIP(hipMemcpyAsync(drf.get(), rf.data(), (nbig + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, s));
-    SME_HIP(hipMemcpyAsync(dwf.get(), wf.data(), (nbig + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, s));
+    SME_HIP(hipMemcpyAsync(dcf_local.get(), cf.data(), (nbig + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, s));
+    SME_HIP(hipMemcpyAsync(dverts_local.get(), verts.data(), nbig * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    SME_HIP(hipMemcpyAsync(drf_local.get(), rf.data(), (nbig + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, s));
+    SME_HIP(hipMemcpyAsync(dwf_local.get(), wf.data(), (nbig + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, s));
+  }
+  if (nbig) {
     // batches of whole vertices whose rows fit what the run may use
     uint64_t cap_words = static_cast<uint64_t>(0.9 * static_cast<double>(ctx->RunAvailable())) / 8;
     if (ctx->tuning.batch_bytes) cap_words = std::min<uint64_t>(cap_words, ctx->tuning.batch_bytes / 8);
@@ -1453,11 +1533,11 @@ void LocalCliqueRange(Context* ctx, const DevGraph* g, const EnginePlan* plan, u
       while (x1 < nbig && wf[x1 + 1] - wf[x0] <= cap_words) ++x1;
       const uint64_t words = wf[x1] - wf[x0];
       Buf<unsigned long long> rows(ctx, words);
-      a.verts = dverts.get() + x0;
+      a.verts = dverts + x0;
       a.nverts = x1 - x0;
       a.nitems = rf[x1] - rf[x0];
-      a.row_first = drf.get() + x0;
-      a.word_first = dwf.get() + x0;
+      a.row_first = drf + x0;
+      a.word_first = dwf + x0;
       a.rows = rows.get();
       {
         // per row: the row vertex, its list bounds (20 bytes) and its words
@@ -1489,11 +1569,11 @@ void LocalCliqueRange(Context* ctx, const DevGraph* g, const EnginePlan* plan, u
           const uint64_t xa = cls ? small_from : hash_from, xb = cls ? x1 : small_from;
           if (xb <= xa) continue;
           CliqueArgs b = a;
-          b.verts = dverts.get() + xa;
+          b.verts = dverts + xa;
           b.nverts = xb - xa;
-          b.row_first = drf.get() + xa;
-          b.word_first = dwf.get() + xa;
-          b.chunk_first = dcf.get() + xa;
+          b.row_first = drf + xa;
+          b.word_first = dwf + xa;
+          b.chunk_first = dcf + xa;
           b.nchunks = cf[xb] - cf[xa];
           b.rows = rows.get() + (wf[xa] - wf[x0]);
           LaunchCliqueRowsHash(b, cls == 1, s);
@@ -1512,7 +1592,9 @@ void LocalCliqueRange(Context* ctx, const DevGraph* g, const EnginePlan* plan, u
   SME_HIP(hipMemcpyAsync(pin, acc.get(), kAcc * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
   const uint64_t nfirst = hh.size();  // start vertices too large to batch
   hh.resize(nfirst + nhuge);
-  if (nhuge)
+  if (nhuge && whole)
+    std::copy(g->upper.huge.begin(), g->upper.huge.end(), hh.begin() + static_cast<int64_t>(nfirst));
+  else if (nhuge)
     SME_HIP(hipMemcpyAsync(hh.data() + nfirst, huge.get(), nhuge * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   SME_HIP(hipStreamSynchronize(s));
   st->d2h_syncs++;
@@ -1522,7 +1604,7 @@ void LocalCliqueRange(Context* ctx, const DevGraph* g, const EnginePlan* plan, u
   // 2-cliques included)
   for (uint64_t x = 0; x < nfirst; ++x) {
     cl[1] -= 1;
-    cl[2] -= hd[order[x]];
+    cl[2] -= hd_sorted[x];
   }
   uint64_t total = cl[K];
   if (levels)

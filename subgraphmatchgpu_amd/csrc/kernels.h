@@ -354,7 +354,8 @@ struct CliqueArgs {
   // CliqueSmallKernel: start vertices [v_begin, v_end).  It counts those
   // with d <= kLocalSmall and adds acc[1] and acc[2] (1 and d) for every
   // vertex with d <= kLocalMax; vertices with kLocalSmall < d <= kLocalMax
-  // are appended to (big, big_d), those with d > kLocalMax to huge.
+  // are appended to (big, big_d), those with d > kLocalMax to huge (not when
+  // big / huge are null: the caller holds these lists already).
   uint32_t v_begin, v_end;
   uint32_t* big;
   uint32_t* big_d;
