@@ -288,6 +288,9 @@ __global__ __launch_bounds__(kThreads) void CliqueRowsHashKernel(CliqueArgs a) {
   __shared__ uint32_t lds_key[kSlots];
   __shared__ uint16_t lds_idx[kSlots];
   __shared__ u64 lds_row[kWaves][kRowsHashMax / 64];
+  // per-edge counts (tc): the chunk's triangles through edge (v, u_j),
+  // flushed with one atomic per j
+  __shared__ uint32_t lds_col[kSlots / 2];
   const int lane = threadIdx.x & (kWave - 1);
   const int wid = threadIdx.x / kWave;
   u64* row = lds_row[wid];
@@ -303,6 +306,8 @@ __global__ __launch_bounds__(kThreads) void CliqueRowsHashKernel(CliqueArgs a) {
     u64* rows = a.rows + (a.word_first[x] - a.word_first[0]);
     __syncthreads();  // the previous chunk's lookups are done
     for (uint32_t k = threadIdx.x; k < kSlots; k += kThreads) lds_key[k] = kEmptyKey;
+    if (a.tc)
+      for (uint32_t k = threadIdx.x; k < d; k += kThreads) lds_col[k] = 0;
     __syncthreads();
     for (uint32_t j = threadIdx.x; j < d; j += kThreads) {
       const uint32_t w = a.cols[nb + j];
@@ -333,8 +338,8 @@ __global__ __launch_bounds__(kThreads) void CliqueRowsHashKernel(CliqueArgs a) {
             atomicOr(row + (j >> 6), 1ull << (j & 63));
             ++bits;
             if (a.tc) {
-              atomicAdd(a.tc + e, 1u);       // edge (u_i, u_j)
-              atomicAdd(a.tc + nb + j, 1u);  // edge (v, u_j)
+              atomicAdd(a.tc + e, 1u);     // edge (u_i, u_j)
+              atomicAdd(lds_col + j, 1u);  // edge (v, u_j)
             }
           }
         }
@@ -347,6 +352,11 @@ __global__ __launch_bounds__(kThreads) void CliqueRowsHashKernel(CliqueArgs a) {
         c3 += tot;
       }
       Sync();
+    }
+    if (a.tc) {
+      __syncthreads();
+      for (uint32_t j = threadIdx.x; j < d; j += kThreads)
+        if (lds_col[j]) atomicAdd(a.tc + nb + j, lds_col[j]);
     }
   }
   if (a.tc) Flush(a.acc, 3, c3, lane);
@@ -365,7 +375,7 @@ __device__ __forceinline__ u64 Probe(const u64* s, uint32_t W, uint32_t d, const
 // Sum over the members j of s (words [lo, W), in LDS) of |s & row_j|, as
 // per-lane partial sums.  Members are queued in LDS (q: 2 x 64 entries) so
 // every probe round has 64 of them, whatever the density of s.
-__device__ u64 LastLevel(const u64* s, uint32_t lo, uint32_t W, uint32_t d, const u64* rows, uint32_t* q,
+__device__ __forceinline__ u64 LastLevel(const u64* s, uint32_t lo, uint32_t W, uint32_t d, const u64* rows, uint32_t* q,
                          int lane) {
   const u64 below = lane ? (~0ull >> (64 - lane)) : 0ull;
   u64 sum = 0;
@@ -394,10 +404,91 @@ __device__ u64 LastLevel(const u64* s, uint32_t lo, uint32_t W, uint32_t d, cons
   return sum;
 }
 
-// One wave per row i of a big vertex: s1 = row i, and for K >= 5 every
-// s2 = s1 & row j (j in s1), s3 = s2 & row l, each kept in LDS while the
-// members below it are walked; the last set's members are probed by
-// LastLevel.
+struct CliqueSums {
+  u64 c3 = 0, c4 = 0, c5 = 0, c6 = 0;
+};
+
+// The cliques from row i of a vertex with d upper neighbours whose rows are
+// stored word-major at `rows` (device memory or LDS): s1 = row i, and for
+// K >= 5 every s2 = s1 & row j (j in s1), s3 = s2 & row l, each kept in the
+// wave's LDS buffers while the members below it are walked; the last set's
+// members are probed by LastLevel.  Adds the level counts to *cs and
+// returns this lane's share of the k-cliques.
+template <int K>
+__device__ __forceinline__ u64 CountRow(const u64* rows, uint32_t d, uint32_t i, u64* s1, u64* s2, u64* s3,
+                                        uint32_t* q, int lane, CliqueSums* cs) {
+  const uint32_t W = (d + 63) / 64;
+  const u64* ri = rows + i;
+  const uint32_t lo1 = (i + 1) >> 6;
+  u64 vk = 0;
+  Sync();
+  for (uint32_t w = lo1 + lane; w < W; w += kWave) {
+    const u64 xw = ri[static_cast<uint64_t>(w) * d];
+    if (K >= 4) s1[w] = xw;
+    cs->c3 += __popcll(xw);
+    if (K == 3) vk += __popcll(xw);
+  }
+  if (K < 4) return vk;
+  Sync();
+  if (K == 4) {
+    const u64 y = LastLevel(s1, lo1, W, d, rows, q, lane);
+    cs->c4 += y;
+    return vk + y;
+  }
+  for (uint32_t c = lo1; c < W; ++c) {
+    for (u64 m1 = First64(s1[c]); m1; m1 &= m1 - 1) {
+      const uint32_t j = c * 64 + static_cast<uint32_t>(__builtin_ctzll(m1));
+      const u64* rj = rows + j;
+      const uint32_t lo2 = (j + 1) >> 6;
+      u64 nz = 0;
+      Sync();
+      for (uint32_t w = lo2 + lane; w < W; w += kWave) {
+        const u64 xw = s1[w] & rj[static_cast<uint64_t>(w) * d];
+        s2[w] = xw;
+        cs->c4 += __popcll(xw);
+        nz |= xw;
+      }
+      if (!__ballot(nz != 0)) continue;
+      Sync();
+      if (K == 5) {
+        const u64 y = LastLevel(s2, lo2, W, d, rows, q, lane);
+        cs->c5 += y;
+        vk += y;
+        continue;
+      }
+      for (uint32_t c2 = lo2; c2 < W; ++c2) {
+        for (u64 m2 = First64(s2[c2]); m2; m2 &= m2 - 1) {
+          const uint32_t l = c2 * 64 + static_cast<uint32_t>(__builtin_ctzll(m2));
+          const u64* rl = rows + l;
+          const uint32_t lo3 = (l + 1) >> 6;
+          u64 nz3 = 0;
+          Sync();
+          for (uint32_t w = lo3 + lane; w < W; w += kWave) {
+            const u64 xw = s2[w] & rl[static_cast<uint64_t>(w) * d];
+            s3[w] = xw;
+            cs->c5 += __popcll(xw);
+            nz3 |= xw;
+          }
+          if (!__ballot(nz3 != 0)) continue;
+          Sync();
+          const u64 y = LastLevel(s3, lo3, W, d, rows, q, lane);
+          cs->c6 += y;
+          vk += y;
+        }
+      }
+    }
+  }
+  return vk;
+}
+
+__device__ __forceinline__ void FlushSums(unsigned long long* acc, int K, CliqueSums* cs, int lane) {
+  Flush(acc, 3, cs->c3, lane);
+  if (K >= 4) Flush(acc, 4, cs->c4, lane);
+  if (K >= 5) Flush(acc, 5, cs->c5, lane);
+  if (K >= 6) Flush(acc, 6, cs->c6, lane);
+}
+
+// One wave per row i of a big vertex, rows in device memory.
 template <int K>
 __global__ __launch_bounds__(kThreads) void CliqueCountKernel(CliqueArgs a) {
   constexpr int kBufs = K >= 4 ? K - 3 : 1;
@@ -411,81 +502,16 @@ __global__ __launch_bounds__(kThreads) void CliqueCountKernel(CliqueArgs a) {
   uint32_t* q = lds_q[wid];
   const uint64_t wave = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) / kWave;
   const uint64_t nwaves = static_cast<uint64_t>(gridDim.x) * blockDim.x / kWave;
-  u64 c3 = 0, c4 = 0, c5 = 0, c6 = 0;
+  CliqueSums cs;
   for (uint64_t t = wave; t < a.nitems; t += nwaves) {
     const uint64_t x = ItemVertex(a, t);
     const uint32_t d = static_cast<uint32_t>(a.row_first[x + 1] - a.row_first[x]);
     const uint32_t i = static_cast<uint32_t>(t - (a.row_first[x] - a.row_first[0]));
-    const uint32_t W = (d + 63) / 64;
     const u64* rows = a.rows + (a.word_first[x] - a.word_first[0]);
-    const u64* ri = rows + i;
-    const uint32_t lo1 = (i + 1) >> 6;
-    u64 vk = 0;
-    Sync();
-    for (uint32_t w = lo1 + lane; w < W; w += kWave) {
-      const u64 xw = ri[static_cast<uint64_t>(w) * d];
-      if (K >= 4) s1[w] = xw;
-      c3 += __popcll(xw);
-      if (K == 3) vk += __popcll(xw);
-    }
-    if (K >= 4) {
-      Sync();
-      if (K == 4) {
-        const u64 y = LastLevel(s1, lo1, W, d, rows, q, lane);
-        c4 += y;
-        vk += y;
-      } else {
-        for (uint32_t c = lo1; c < W; ++c) {
-          for (u64 m1 = First64(s1[c]); m1; m1 &= m1 - 1) {
-            const uint32_t j = c * 64 + static_cast<uint32_t>(__builtin_ctzll(m1));
-            const u64* rj = rows + j;
-            const uint32_t lo2 = (j + 1) >> 6;
-            u64 nz = 0;
-            Sync();
-            for (uint32_t w = lo2 + lane; w < W; w += kWave) {
-              const u64 xw = s1[w] & rj[static_cast<uint64_t>(w) * d];
-              s2[w] = xw;
-              c4 += __popcll(xw);
-              nz |= xw;
-            }
-            if (!__ballot(nz != 0)) continue;
-            Sync();
-            if (K == 5) {
-              const u64 y = LastLevel(s2, lo2, W, d, rows, q, lane);
-              c5 += y;
-              vk += y;
-              continue;
-            }
-            for (uint32_t c2 = lo2; c2 < W; ++c2) {
-              for (u64 m2 = First64(s2[c2]); m2; m2 &= m2 - 1) {
-                const uint32_t l = c2 * 64 + static_cast<uint32_t>(__builtin_ctzll(m2));
-                const u64* rl = rows + l;
-                const uint32_t lo3 = (l + 1) >> 6;
-                u64 nz3 = 0;
-                Sync();
-                for (uint32_t w = lo3 + lane; w < W; w += kWave) {
-                  const u64 xw = s2[w] & rl[static_cast<uint64_t>(w) * d];
-                  s3[w] = xw;
-                  c5 += __popcll(xw);
-                  nz3 |= xw;
-                }
-                if (!__ballot(nz3 != 0)) continue;
-                Sync();
-                const u64 y = LastLevel(s3, lo3, W, d, rows, q, lane);
-                c6 += y;
-                vk += y;
-              }
-            }
-          }
-        }
-      }
-    }
+    const u64 vk = CountRow<K>(rows, d, i, s1, s2, s3, q, lane, &cs);
     if (a.bk.bounds) AddBucket(a.bk, a.verts[x], vk, lane);
   }
-  Flush(a.acc, 3, c3, lane);
-  if (K >= 4) Flush(a.acc, 4, c4, lane);
-  if (K >= 5) Flush(a.acc, 5, c5, lane);
-  if (K >= 6) Flush(a.acc, 6, c6, lane);
+  FlushSums(a.acc, K, &cs, lane);
 }
 
 }  // namespace

@@ -1539,6 +1539,15 @@ void LocalCliqueRange(Context* ctx, const DevGraph* g, const EnginePlan* plan, u
       a.row_first = drf + x0;
       a.word_first = dwf + x0;
       a.rows = rows.get();
+      // the batch is in decreasing d: vertices above kRowsHashMax search,
+      // the others stream against an LDS hash (a small table up to
+      // kRowsHashSmall)
+      const uint64_t hash_from = std::partition_point(hd_sorted.begin() + x0, hd_sorted.begin() + x1,
+                                                      [](uint32_t d) { return d > kRowsHashMax; }) -
+                                 hd_sorted.begin();
+      const uint64_t small_from = std::partition_point(hd_sorted.begin() + hash_from, hd_sorted.begin() + x1,
+                                                       [](uint32_t d) { return d > kRowsHashSmall; }) -
+                                  hd_sorted.begin();
       {
         // per row: the row vertex, its list bounds (20 bytes) and its words
         // written; per later neighbour of the start vertex: the key and one
@@ -1550,15 +1559,6 @@ void LocalCliqueRange(Context* ctx, const DevGraph* g, const EnginePlan* plan, u
         }
         Timed t(ctx, s, kSlotCliqueRows,
                 8.0 * keys + 20.0 * static_cast<double>(a.nitems) + 8.0 * static_cast<double>(words), a.nitems);
-        // the batch is in decreasing d: vertices above kRowsHashMax search,
-        // the others stream against an LDS hash (a small table up to
-        // kRowsHashSmall)
-        const uint64_t hash_from = std::partition_point(hd_sorted.begin() + x0, hd_sorted.begin() + x1,
-                                                        [](uint32_t d) { return d > kRowsHashMax; }) -
-                                   hd_sorted.begin();
-        const uint64_t small_from = std::partition_point(hd_sorted.begin() + hash_from, hd_sorted.begin() + x1,
-                                                         [](uint32_t d) { return d > kRowsHashSmall; }) -
-                                    hd_sorted.begin();
         if (hash_from > x0) {
           CliqueArgs b = a;
           b.nverts = hash_from - x0;
