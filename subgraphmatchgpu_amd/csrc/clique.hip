@@ -317,9 +317,21 @@ __global__ __launch_bounds__(kThreads) void CliqueRowsHashKernel(CliqueArgs a) {
     }
     __syncthreads();
     const uint32_t vmax = a.cols[nb + d - 1];
-    for (uint32_t i = r0 + wid; i < r1; i += kWaves) {
-      const uint32_t ui = a.cols[nb + i];
-      const uint64_t lb = a.up[ui], le = a.rp[ui + 1];
+    // the next row's vertex and list bounds are loaded while this row streams
+    uint32_t i = r0 + wid;
+    uint64_t lb_next = 0, le_next = 0;
+    if (i < r1) {
+      const uint32_t u = a.cols[nb + i];
+      lb_next = a.up[u];
+      le_next = a.rp[u + 1];
+    }
+    for (; i < r1; i += kWaves) {
+      const uint64_t lb = lb_next, le = le_next;
+      if (i + kWaves < r1) {
+        const uint32_t u = a.cols[nb + i + kWaves];
+        lb_next = a.up[u];
+        le_next = a.rp[u + 1];
+      }
       for (uint32_t w = lane; w < W; w += kWave) row[w] = 0;
       Sync();
       uint32_t bits = 0;
