@@ -225,6 +225,8 @@ def main():
     ap.add_argument("--tune", default="0,0,0,4,0,512",
                     help="lane_path_max,item_grid,batch_bytes,super_chunks[,stream_ratio[,group_max]] "
                          "schedule (rmat20)")
+    ap.add_argument("--level-engine-steps", type=int, default=2,
+                    help="rmat20: also time this many steps through the level engine (0: skip)")
     ap.add_argument("--no-rank-order", action="store_true",
                     help="rmat20: no degree-ranked copy (clique counts then walk the original ids)")
     ap.add_argument("--summarize-pmc", nargs="+", metavar="ARG",
@@ -379,6 +381,43 @@ def main():
             },
         },
     }
+    # the same step through the level engine (clique and census paths off):
+    # its intersection and scatter kernels, measured the same way
+    if args.level_engine_steps:
+        sched = dev.get_tuning()
+        dev.set_tuning(**dict(sched, clique_level_engine=1, census_level_engine=1))
+        step()  # warm-up
+        dev.profile_reset()
+        dev.profile(True)
+        t2 = time.perf_counter()
+        for _ in range(args.level_engine_steps):
+            if step() != counts:
+                raise AssertionError("the level engine's counts differ")
+        le_elapsed = time.perf_counter() - t2
+        dev.profile(False)
+        lprof = dev.profile_read()
+        dev.set_tuning(**sched)
+
+        def level_family(names):
+            ms = sum(lprof[k]["ms"] for k in names)
+            by = sum(lprof[k]["bytes"] for k in names)
+            gbs = (by / 1e9) / (ms / 1e3) if ms > 0 else 0.0
+            return {"kernels": list(names), "launches": sum(lprof[k]["launches"] for k in names),
+                    "ms_per_step": ms / args.level_engine_steps,
+                    "GB_per_step": by / 1e9 / args.level_engine_steps, "achieved": gbs,
+                    "frac": gbs / HBM_PEAK_GBS}
+
+        out["level_engine"] = {
+            "what": "the same step with clique_level_engine = census_level_engine = 1 (LIGHT level engine: "
+                    "group counting for the 4-clique's last level, item verify kernels, scan and scatter); "
+                    "counts checked equal",
+            "ms_per_step": 1e3 * le_elapsed / args.level_engine_steps,
+            "value": sum(counts) * args.level_engine_steps / le_elapsed,
+            "families": {
+                "intersect_all": level_family(("intersect", "intersect_group", "intersect_lane")),
+                "scatter": level_family(("expand", "compact")),
+            },
+        }
     # HBM bytes per launch of the dominant kernel measured by rocprofv3
     # counters for this same configuration (TRAFFIC_FILE)
     try:
