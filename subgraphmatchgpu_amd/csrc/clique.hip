@@ -515,13 +515,24 @@ __global__ __launch_bounds__(kThreads) void CliqueCountKernel(CliqueArgs a) {
   const uint64_t wave = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) / kWave;
   const uint64_t nwaves = static_cast<uint64_t>(gridDim.x) * blockDim.x / kWave;
   CliqueSums cs;
-  for (uint64_t t = wave; t < a.nitems; t += nwaves) {
-    const uint64_t x = ItemVertex(a, t);
-    const uint32_t d = static_cast<uint32_t>(a.row_first[x + 1] - a.row_first[x]);
-    const uint32_t i = static_cast<uint32_t>(t - (a.row_first[x] - a.row_first[0]));
-    const u64* rows = a.rows + (a.word_first[x] - a.word_first[0]);
-    const u64 vk = CountRow<K>(rows, d, i, s1, s2, s3, q, lane, &cs);
-    if (a.bk.bounds) AddBucket(a.bk, a.verts[x], vk, lane);
+  // up to 4-cliques a row is light next to the search for its vertex: a
+  // wave takes groups of kRowGroup consecutive rows, one search for the
+  // group's first vertex, then a walk forward.  Deeper cliques' rows are
+  // heavy, and the first rows of the largest vertices heaviest; one row at
+  // a time keeps them spread over the waves.
+  constexpr uint64_t kRowGroup = K <= 4 ? 16 : 1;
+  const uint64_t base = a.row_first[0];
+  for (uint64_t t0 = wave * kRowGroup; t0 < a.nitems; t0 += nwaves * kRowGroup) {
+    const uint64_t te = t0 + kRowGroup < a.nitems ? t0 + kRowGroup : a.nitems;
+    uint64_t x = ItemVertex(a, t0);
+    for (uint64_t t = t0; t < te; ++t) {
+      while (a.row_first[x + 1] - base <= t) ++x;
+      const uint32_t d = static_cast<uint32_t>(a.row_first[x + 1] - a.row_first[x]);
+      const uint32_t i = static_cast<uint32_t>(t - (a.row_first[x] - base));
+      const u64* rows = a.rows + (a.word_first[x] - a.word_first[0]);
+      const u64 vk = CountRow<K>(rows, d, i, s1, s2, s3, q, lane, &cs);
+      if (a.bk.bounds) AddBucket(a.bk, a.verts[x], vk, lane);
+    }
   }
   FlushSums(a.acc, K, &cs, lane);
 }
