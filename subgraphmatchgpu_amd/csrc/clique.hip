@@ -335,13 +335,22 @@ __global__ __launch_bounds__(kThreads) void CliqueRowsHashKernel(CliqueArgs a) {
       for (uint32_t w = lane; w < W; w += kWave) row[w] = 0;
       Sync();
       uint32_t bits = 0;
-      for (uint64_t e0 = lb; e0 < le; e0 += kWave) {
+      // chunks of 64 entries, the next chunk's load issued before this
+      // chunk's lookups
+      uint64_t e0 = lb;
+      uint32_t wcur = lb + lane < le ? a.cols[lb + lane] : kEmptyKey;
+      while (e0 < le) {
         // N+(u_i) is sorted: nothing from a chunk starting above the
-        // largest value of N+(v) on can be found
-        if (static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(a.cols[e0]))) > vmax) break;
+        // largest value of N+(v) on can be found (lane 0 holds the chunk's
+        // first entry)
+        if (static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(wcur))) > vmax) break;
+        const uint64_t n0 = e0 + kWave;
+        const uint32_t wnext = n0 + lane < le ? a.cols[n0 + lane] : kEmptyKey;
         const uint64_t e = e0 + lane;
+        const uint32_t w = wcur;
+        e0 = n0;
+        wcur = wnext;
         if (e < le) {
-          const uint32_t w = a.cols[e];
           uint32_t h = (w * 0x9E3779B1u) >> (32 - kBits);
           uint32_t k;
           while ((k = lds_key[h]) != w && k != kEmptyKey) h = (h + 1) & (kSlots - 1);
