@@ -156,7 +156,7 @@ KERNEL_NAMES = {
 }
 # Revision of the intersect kernel's memory behaviour; a committed traffic
 # figure is used only when it was measured on the same revision.
-KERNEL_REV = "clique-bitmaps-hash-rows-word-major-ldscol"
+KERNEL_REV = "clique-bitmaps-hash-rows-word-major-ldscol-pipelined"
 
 
 def summarize_pmc(fetch_csv, write_csv, calib_bytes, kernel="CliqueRowsHashKernel", per_launch=1):

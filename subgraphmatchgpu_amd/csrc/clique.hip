@@ -143,17 +143,51 @@ __global__ __launch_bounds__(kThreads) void CliqueSmallKernel(CliqueArgs a) {
     const uint32_t uj = static_cast<uint32_t>(lane) < du ? a.cols[nb + lane] : 0u;
     u64 row = 0;     // lane i: row i
     uint32_t col = 0;  // lane j: rows i < j holding bit j
-    for (uint32_t i = 0; i + 1 < du; ++i) {
-      const uint32_t ui = Lane32(uj, i);
-      const uint64_t lb = a.up[ui], le = a.rp[ui + 1];
-      uint64_t pos = 0;
-      const bool ok = static_cast<uint32_t>(lane) > i && static_cast<uint32_t>(lane) < du &&
-                      Holds(a.cols, lb, le, uj, &pos);
-      const u64 r = __ballot(ok);
-      if (static_cast<uint32_t>(lane) == i) row = r;
-      if (ok && a.tc) {
-        atomicAdd(a.tc + pos, 1u);  // edge (u_i, u_j): triangle (v, u_i, u_j)
-        ++col;
+    // rows i and i + 1 together: their two searches run in lockstep, so two
+    // independent loads are in flight per step
+    for (uint32_t i = 0; i + 1 < du; i += 2) {
+      const uint32_t ua = Lane32(uj, i);
+      const uint32_t ub = Lane32(uj, i + 1);
+      uint64_t lo1 = a.up[ua], hi1 = a.rp[ua + 1], lo2 = a.up[ub], hi2 = a.rp[ub + 1];
+      const uint64_t end1 = hi1, end2 = hi2;
+      const bool act1 = static_cast<uint32_t>(lane) > i && static_cast<uint32_t>(lane) < du;
+      const bool act2 = static_cast<uint32_t>(lane) > i + 1 && static_cast<uint32_t>(lane) < du;
+      if (!act1) lo1 = hi1;
+      if (!act2) lo2 = hi2;
+      while (lo1 < hi1 || lo2 < hi2) {
+        const bool g1 = lo1 < hi1, g2 = lo2 < hi2;
+        const uint64_t m1 = (lo1 + hi1) >> 1, m2 = (lo2 + hi2) >> 1;
+        const uint32_t x1 = g1 ? a.cols[m1] : 0u;
+        const uint32_t x2 = g2 ? a.cols[m2] : 0u;
+        if (g1) {
+          if (x1 < uj)
+            lo1 = m1 + 1;
+          else
+            hi1 = m1;
+        }
+        if (g2) {
+          if (x2 < uj)
+            lo2 = m2 + 1;
+          else
+            hi2 = m2;
+        }
+      }
+      const bool ok1 = act1 && lo1 < end1 && a.cols[lo1] == uj;
+      const bool ok2 = act2 && lo2 < end2 && a.cols[lo2] == uj;
+      const u64 r1 = __ballot(ok1);
+      const u64 r2 = __ballot(ok2);
+      if (static_cast<uint32_t>(lane) == i) row = r1;
+      if (static_cast<uint32_t>(lane) == i + 1) row = r2;
+      if (a.tc) {
+        // edges (u_i, u_j): triangles (v, u_i, u_j)
+        if (ok1) {
+          atomicAdd(a.tc + lo1, 1u);
+          ++col;
+        }
+        if (ok2) {
+          atomicAdd(a.tc + lo2, 1u);
+          ++col;
+        }
       }
     }
     const u64 tri = __popcll(row);
