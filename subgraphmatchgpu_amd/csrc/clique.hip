@@ -401,10 +401,15 @@ __global__ __launch_bounds__(kThreads) void CliqueRowsHashKernel(CliqueArgs a) {
       }
       Sync();
       for (uint32_t w = lane; w < W; w += kWave) rows[static_cast<uint64_t>(w) * d + i] = row[w];
-      const u64 tot = Sum64(bits);
-      if (lane == 0) {
-        if (a.tc && tot) atomicAdd(a.tc + nb + i, static_cast<uint32_t>(tot));  // edge (v, u_i)
-        c3 += tot;
+      if (a.tc) {
+        // edge (v, u_i): one triangle per bit of row i (only the per-edge
+        // pass needs the row's popcount here; otherwise the count kernel
+        // takes it)
+        const u64 tot = Sum64(bits);
+        if (lane == 0) {
+          if (tot) atomicAdd(a.tc + nb + i, static_cast<uint32_t>(tot));
+          c3 += tot;
+        }
       }
       Sync();
     }
